@@ -1,0 +1,1173 @@
+// MI355X (gfx950) expansion engine for maximal directed quasi-clique enumeration.
+//
+// One 256-thread workgroup owns one task at a time and runs its branch loop; the
+// workgroups are persistent and pull tasks from the current batch with one atomic.
+// All tasks live on one LIFO task stack in HBM: each launch expands a batch popped from
+// the stack top, and children are pushed back onto the top with one combined 64-bit
+// atomic per child (descriptor count and record count move together, so the records
+// of consecutive descriptors are contiguous). Popping the newest tasks first keeps the
+// stack close to depth-first order, which bounds its size the way the reference's LIFO
+// buffer does (src/device_kernels.cu:300-326).
+//
+// Per workgroup state:
+//   * LDS bitmaps over condensed vertex ids: parent-live, parent-member, child-member,
+//     child-candidate, in-L(v), critical. Every "is this neighbour in the task" test
+//     of the reference's vertex_order_map (src/device_kernels.cu:433-435) becomes one
+//     LDS bit test;
+//   * HBM arrays indexed by condensed id for the parent's and the child's degrees, so
+//     scatters need no position lookup;
+//   * an LDS bitonic sort of each child on a packed 64-bit enumeration-order key.
+//
+// Semantics follow the reference's per-task branch loop exactly (host level 0 / 1 or
+// device levels, selected per task); the functions cite the reference lines they follow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dqc.h"
+#include "graph.hpp"
+
+namespace {
+
+constexpr int NT = 256;            // threads per workgroup
+constexpr int KW = 512;            // bitmap words: condensed ids < 16384
+constexpr int KMAX = KW * 32;
+constexpr int SORT_N = 4096;       // LDS sort capacity (children larger use the fallback)
+constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
+constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2;
+constexpr unsigned long long CNT_SHIFT = 40;  // combined allocation: (count << 40) | size
+constexpr unsigned long long SIZE_MASK = (1ull << CNT_SHIFT) - 1;
+
+enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u };
+
+struct TaskDesc {
+    unsigned long long off;
+    int32_t tot, nm, ncov, mode;
+};
+
+struct Counters {
+    unsigned long long next_task;
+    unsigned long long sp_alloc;   // task stack top: (tasks << 40) | records
+    unsigned long long clq_alloc;  // clique store: (sets << 40) | ids
+    unsigned long long tasks, branches, cliques, bytes, slow_sorts;
+    unsigned int error;
+    unsigned int pad;
+};
+
+struct KParams {
+    int32_t n;
+    const unsigned long long *oo, *io, *to;
+    const int32_t *on, *in, *tn;
+    const int32_t *mdo, *mdi, *ubo, *ubi;
+    int32_t min_size, host_addfail;
+    const TaskDesc* in_desc;
+    const dqc::TaskVertex* in_vtx;
+    unsigned long long in_ntask;
+    TaskDesc* sp_desc;
+    dqc::TaskVertex* sp_vtx;
+    unsigned long long sp_cap_t, sp_cap_v;
+    int32_t* clq_ids;
+    unsigned long long* clq_meta;  // (start << 24) | len per set
+    unsigned long long clq_cap_s, clq_cap_i;
+    Counters* ctr;
+    uint32_t* scratch;
+    unsigned long long stride;  // uint32 words per workgroup
+};
+
+struct Shared {
+    uint32_t plive[KW], pmem[KW], cmem[KW], ccand[KW], cL[KW], ccrit[KW];
+    unsigned long long sortk[SORT_N];
+    int32_t ho[HMAX + 2], hi[HMAX + 2], so[HMAX + 2], si[HMAX + 2];
+    int32_t cnt[8];
+    int32_t red[8];
+    int32_t lu[12];
+    unsigned long long bcast;
+    unsigned long long acc_bytes;
+};
+
+__device__ __forceinline__ bool bt(const uint32_t* bm, int id) { return (bm[id >> 5] >> (id & 31)) & 1u; }
+__device__ __forceinline__ void bs(uint32_t* bm, int id) { atomicOr(&bm[id >> 5], 1u << (id & 31)); }
+__device__ __forceinline__ void bcl(uint32_t* bm, int id) { atomicAnd(&bm[id >> 5], ~(1u << (id & 31))); }
+
+__device__ __forceinline__ int bitw(unsigned long long x) { return x == 0 ? 0 : 64 - __clzll(x); }
+
+// All functions below are executed by every thread of the workgroup with uniform
+// control flow; `W` bundles the per-workgroup pointers and uniform scalars.
+struct W {
+    const KParams* P;
+    Shared* S;
+    int tid;
+    int kw;  // bitmap words in use
+    uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
+    int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
+    unsigned long long bytes;  // per-thread traffic tally (CSR + records)
+
+    __device__ int gmd(int k, const int32_t* md) const { return k < P->min_size ? md[P->min_size] : md[k]; }
+
+    // one broadcast of a value computed by thread 0 (two barriers: write, then read)
+    __device__ unsigned long long bcast0(unsigned long long v) {
+        if (tid == 0) S->bcast = v;
+        __syncthreads();
+        unsigned long long r = S->bcast;
+        __syncthreads();
+        return r;
+    }
+    __device__ int read_counter(int slot) {
+        __syncthreads();
+        int v = S->cnt[slot];
+        __syncthreads();
+        return v;
+    }
+
+    // ---------------------------------------------------------------- predicates
+    // reference inc/device_kernels.hpp:111-140 (d_cand_isvalid); unsigned compares kept
+    __device__ bool cand_valid(int w, int cs, int ub, int lb, int meo, int mei) const {
+        const int32_t *mdo = P->mdo, *mdi = P->mdi;
+        uint32_t om = com[w], oc = coc[w], im = cim[w], ic = cic[w];
+        if (om + oc < (uint32_t)meo) return false;
+        if (im + ic < (uint32_t)mei) return false;
+        if (om + oc < (uint32_t)gmd((int)(cs + oc + 1), mdo)) return false;
+        if (im + ic < (uint32_t)gmd((int)(cs + ic + 1), mdi)) return false;
+        if (om + (uint32_t)ub - 1u < (uint32_t)mdo[cs + ub]) return false;
+        if (im + (uint32_t)ub - 1u < (uint32_t)mdi[cs + ub]) return false;
+        if (om + oc < (uint32_t)gmd(cs + lb, mdo)) return false;
+        if (im + ic < (uint32_t)gmd(cs + lb, mdi)) return false;
+        return true;
+    }
+    // reference inc/device_kernels.hpp:141-178 (d_vert_isextendable)
+    __device__ bool vert_ext(int w, int cs, int ub, int lb, int meo, int mei) const {
+        const int32_t *mdo = P->mdo, *mdi = P->mdi;
+        uint32_t om = com[w], oc = coc[w], im = cim[w], ic = cic[w];
+        if (om + oc < (uint32_t)meo) return false;
+        if (im + ic < (uint32_t)mei) return false;
+        if (om + oc < (uint32_t)gmd((int)(cs + oc), mdo)) return false;
+        if (im + ic < (uint32_t)gmd((int)(cs + ic), mdi)) return false;
+        if (oc == 0 && om < (uint32_t)gmd(cs + 1, mdo)) return false;
+        if (ic == 0 && im < (uint32_t)gmd(cs + 1, mdi)) return false;
+        if (om + (uint32_t)ub < (uint32_t)mdo[cs + ub]) return false;
+        if (im + (uint32_t)ub < (uint32_t)mdi[cs + ub]) return false;
+        if (om + oc < (uint32_t)gmd(cs + lb, mdo)) return false;
+        if (im + ic < (uint32_t)gmd(cs + lb, mdi)) return false;
+        return true;
+    }
+
+    // ---------------------------------------------------------------- LU bounds
+    // Sum of the i largest candidate member degrees, from the value tables built by
+    // lu_tables (start index and sum-above per value, values 0..cm).
+    __device__ int psum(const int32_t* st, const int32_t* sa, int cm, int i) const {
+        int lo = 0, hi = cm;  // smallest x with st[x] <= i
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (st[mid] <= i) hi = mid; else lo = mid + 1;
+        }
+        return sa[lo] + (i - st[lo]) * lo;
+    }
+
+    // smallest i in [a, b] with pred(i), else b + 1 (a > b gives a)
+    template <class F>
+    __device__ int first_true(int a, int b, F pred) {
+        for (int base = a; base <= b; base += NT) {
+            int i = base + tid;
+            int p = (i <= b) && pred(i);
+            if (tid == 0) S->red[0] = INT_MAX;
+            __syncthreads();
+            if (p) atomicMin(&S->red[0], i);
+            __syncthreads();
+            int r = S->red[0];
+            __syncthreads();
+            if (r != INT_MAX) return r;
+        }
+        return a > b ? a : b + 1;
+    }
+    // largest i in [a, b] with pred(i), else a - 1
+    template <class F>
+    __device__ int last_true(int a, int b, F pred) {
+        for (int top = b; top >= a; top -= NT) {
+            int i = top - tid;
+            int p = (i >= a) && pred(i);
+            if (tid == 0) S->red[1] = INT_MIN;
+            __syncthreads();
+            if (p) atomicMax(&S->red[1], i);
+            __syncthreads();
+            int r = S->red[1];
+            __syncthreads();
+            if (r != INT_MIN) return r;
+        }
+        return a - 1;
+    }
+
+    // reference src/device_kernels.cu:1307-1486 (d_calculate_LU_bounds), with the
+    // member scan on thread 0 and the candidate-prefix loops as parallel searches over
+    // the sorted candidate member degrees (held as histograms in S->ho / S->hi).
+    __device__ bool lu_bounds(int cm, int rc, int& ub, int& lb, int& meo, int& mei) {
+        const int32_t *mdo = P->mdo, *mdi = P->mdi;
+        if (tid == 0) {
+            int x = mem[0];
+            int mino = (int)com[x], mino_c = (int)coc[x], sumo = (int)com[x], mint_o = (int)(com[x] + coc[x]);
+            int mini = (int)cim[x], mini_c = (int)cic[x], sumi = (int)cim[x], mint_i = (int)(cim[x] + cic[x]);
+            for (int a = 1; a < cm; ++a) {
+                int y = mem[a];
+                uint32_t om = com[y], oc = coc[y], im = cim[y], ic = cic[y];
+                sumo += (int)om;
+                if ((uint32_t)mino > om) { mino = (int)om; mino_c = (int)oc; }
+                else if ((uint32_t)mino == om) { if ((uint32_t)mino_c > oc) mino_c = (int)oc; }
+                if ((uint32_t)mint_o > om + oc) mint_o = (int)(om + oc);
+                sumi += (int)im;
+                if ((uint32_t)mini > im) { mini = (int)im; mini_c = (int)ic; }
+                else if ((uint32_t)mini == im) { if ((uint32_t)mini_c > ic) mini_c = (int)ic; }
+                if ((uint32_t)mint_i > im + ic) mint_i = (int)(im + ic);
+            }
+            S->lu[0] = mino; S->lu[1] = mino_c; S->lu[2] = sumo; S->lu[3] = mint_o;
+            S->lu[4] = mini; S->lu[5] = mini_c; S->lu[6] = sumi; S->lu[7] = mint_i;
+            // value tables: st[x] = #values above x, sa[x] = sum of values above x
+            int c = 0, s = 0;
+            for (int v = cm; v >= 0; --v) { int h = S->ho[v]; S->ho[v] = c; S->so[v] = s; c += h; s += h * v; }
+            c = 0; s = 0;
+            for (int v = cm; v >= 0; --v) { int h = S->hi[v]; S->hi[v] = c; S->si[v] = s; c += h; s += h * v; }
+        }
+        __syncthreads();
+        const int mino = S->lu[0], mino_c = S->lu[1], sumo = S->lu[2], mint_o = S->lu[3];
+        const int mini = S->lu[4], mini_c = S->lu[5], sumi = S->lu[6], mint_i = S->lu[7];
+        __syncthreads();
+        meo = gmd(cm + 1, mdo);
+        mei = gmd(cm + 1, mdi);
+        if (mino < mdo[cm] || mini < mdi[cm]) {
+            int nmin = max(gmd(cm, mdo) - mino, gmd(cm, mdi) - mini);
+            int no = first_true(nmin, mino_c, [&](int i) { return !(mino + i < mdo[cm + i]); });
+            if (mino + no < mdo[cm + no]) return false;
+            int ni = first_true(nmin, mini_c, [&](int i) { return !(mini + i < mdi[cm + i]); });
+            if (mini + ni < mdi[cm + ni]) return false;
+            lb = max(no, ni);
+            ub = min(P->ubo[mint_o], P->ubi[mint_i]) + 1 - cm;
+            if (ub > rc) ub = rc;
+            if (lb < ub) {
+                const int32_t *sto = S->ho, *sao = S->so, *sti = S->hi, *sai = S->si;
+                int i1 = first_true(lb, ub - 1, [&](int i) {
+                    int so_ = psum(sto, sao, cm, i), si_ = psum(sti, sai, cm, i);
+                    return !(sumo + si_ < cm * mdo[cm + i] && sumi + so_ < cm * mdi[cm + i]);
+                });
+                {
+                    int so_ = psum(sto, sao, cm, i1), si_ = psum(sti, sai, cm, i1);
+                    if (sumo + so_ < cm * mdo[cm + i1] && sumi + si_ < cm * mdi[cm + i1]) return false;
+                }
+                lb = i1;
+                int t = last_true(i1 + 1, ub, [&](int i) {
+                    int so_ = psum(sto, sao, cm, i), si_ = psum(sti, sai, cm, i);
+                    return sumo + si_ >= cm * mdo[cm + i] && sumi + so_ >= cm * mdi[cm + i];
+                });
+                int tight = t >= i1 + 1 ? t : i1;
+                if (ub > tight) ub = tight;
+                if (lb > 1) { meo = gmd(cm + lb, mdo); mei = gmd(cm + lb, mdi); }
+            }
+        } else {
+            ub = rc;
+            lb = cm < P->min_size ? P->min_size - cm : 0;
+        }
+        if (cm + ub < P->min_size) return false;
+        if (ub < 0 || ub < lb) return false;
+        return true;
+    }
+
+    __device__ void clear_hist(int cm) {
+        for (int v = tid; v <= cm + 1 && v < HMAX + 2; v += NT) { S->ho[v] = 0; S->hi[v] = 0; }
+    }
+
+    __device__ bool members_ext(int cm, int ub, int lb, int meo, int mei) {
+        int bad = 0;
+        for (int a = tid; a < cm; a += NT)
+            if (!vert_ext(mem[a], cm, ub, lb, meo, mei)) bad = 1;
+        return !__syncthreads_or(bad);
+    }
+
+    // Splits R[0..nr) by validity: valid ids to keep[], invalid ones to drop[] (their
+    // candidate bit cleared). Returns the kept count; *nd gets the dropped count.
+    __device__ int filter(const int32_t* R, int nr, int32_t* keep, int32_t* drop, int* nd,
+                          int cm, int ub, int lb, int meo, int mei) {
+        if (tid == 0) { S->cnt[0] = 0; S->cnt[1] = 0; }
+        __syncthreads();
+        for (int i = tid; i < nr; i += NT) {
+            int w = R[i];
+            if (cand_valid(w, cm, ub, lb, meo, mei)) keep[atomicAdd(&S->cnt[0], 1)] = w;
+            else { drop[atomicAdd(&S->cnt[1], 1)] = w; bcl(S->ccand, w); }
+        }
+        __syncthreads();
+        int k = S->cnt[0];
+        *nd = S->cnt[1];
+        __syncthreads();
+        return k;
+    }
+
+    // can-degrees of live vertices recounted as |out(x) n R|, |in(x) n R| (R = ccand)
+    __device__ void recount(int cm, const int32_t* R, int nr) {
+        const unsigned long long *oo = P->oo, *io = P->io;
+        for (int i = tid; i < cm + nr; i += NT) {
+            int x = i < cm ? mem[i] : R[i - cm];
+            uint32_t c = 0;
+            for (unsigned long long e = oo[x]; e < oo[x + 1]; ++e) c += bt(S->ccand, P->on[e]);
+            coc[x] = c;
+            c = 0;
+            for (unsigned long long e = io[x]; e < io[x + 1]; ++e) c += bt(S->ccand, P->in[e]);
+            cic[x] = c;
+            bytes += 4ull * ((oo[x + 1] - oo[x]) + (io[x + 1] - io[x])) + 32;
+        }
+        __syncthreads();
+    }
+    // can-degrees of live vertices decremented by the arcs of dropped candidates
+    __device__ void decrement(const int32_t* D, int nd) {
+        const unsigned long long *oo = P->oo, *io = P->io;
+        for (int i = tid; i < nd; i += NT) {
+            int u = D[i];
+            for (unsigned long long e = oo[u]; e < oo[u + 1]; ++e) {
+                int w = P->on[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&cic[w], 1u);
+            }
+            for (unsigned long long e = io[u]; e < io[u + 1]; ++e) {
+                int w = P->in[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&coc[w], 1u);
+            }
+            bytes += 4ull * ((oo[u + 1] - oo[u]) + (io[u + 1] - io[u])) + 32;
+        }
+        __syncthreads();
+    }
+    // count valid candidates (old bounds) into the member-degree histograms
+    __device__ int hist_valid(const int32_t* R, int nr, int cm, int ub, int lb, int meo, int mei) {
+        clear_hist(cm);
+        __syncthreads();
+        int c = 0;
+        for (int i = tid; i < nr; i += NT) {
+            int w = R[i];
+            if (cand_valid(w, cm, ub, lb, meo, mei)) {
+                atomicAdd(&S->ho[com[w]], 1);
+                atomicAdd(&S->hi[cim[w]], 1);
+                ++c;
+            }
+        }
+        if (tid == 0) S->cnt[2] = 0;
+        __syncthreads();
+        if (c) atomicAdd(&S->cnt[2], c);
+        return read_counter(2);
+    }
+
+    // reference src/device_kernels.cu:965-1305 (d_degree_pruning). On entry the
+    // candidates that passed diameter pruning are R[0..rc) (bits in ccand, degrees in the
+    // histograms); ccount is the child's candidate count before that pruning. Degrees are
+    // recounted exactly over the remaining set (the reference's recount and decrement
+    // branches produce the same values). On success *R / *nr hold the final candidates.
+    __device__ bool degree_prune(int cm, int32_t** R, int* nr, int ccount, int& ub, int& lb,
+                                 int& meo, int& mei, int32_t* spare, int32_t* dropped) {
+        int rc = *nr;
+        if (!lu_bounds(cm, rc, ub, lb, meo, mei)) return false;
+        if (!members_ext(cm, ub, lb, meo, mei)) return false;
+        int nd = 0;
+        int32_t* cur = *R;
+        int k = filter(cur, rc, spare, dropped, &nd, cm, ub, lb, meo, mei);
+        int32_t* tmp = cur; cur = spare; spare = tmp;
+        int removed = ccount - k;
+        bool all_mat = (removed == nd);
+        while (k > 0 && removed > 0) {
+            if (all_mat && nd < k + cm) decrement(dropped, nd);
+            else recount(cm, cur, k);
+            int nv = hist_valid(cur, k, cm, ub, lb, meo, mei);
+            if (!lu_bounds(cm, nv, ub, lb, meo, mei)) { *R = cur; *nr = k; return false; }
+            if (!members_ext(cm, ub, lb, meo, mei)) { *R = cur; *nr = k; return false; }
+            int k2 = filter(cur, k, spare, dropped, &nd, cm, ub, lb, meo, mei);
+            tmp = cur; cur = spare; spare = tmp;
+            removed = nd;
+            all_mat = true;
+            k = k2;
+        }
+        *R = cur;
+        *nr = k;
+        return true;
+    }
+
+    // reference src/device_kernels.cu:1488-1532 (d_check_for_clique)
+    __device__ void check_clique(int cm) {
+        if (cm < P->min_size) return;
+        uint32_t mo = (uint32_t)P->mdo[cm], mi = (uint32_t)P->mdi[cm];
+        int bad = 0;
+        for (int a = tid; a < cm; a += NT) {
+            int x = mem[a];
+            if (com[x] < mo || cim[x] < mi) bad = 1;
+        }
+        if (__syncthreads_or(bad)) return;
+        emit_clique(mem, cm);
+    }
+
+    __device__ void emit_clique(const int32_t* ids, int len) {
+        unsigned long long a = 0;
+        if (tid == 0) a = atomicAdd(&P->ctr->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
+        a = bcast0(a);
+        unsigned long long idx = a >> CNT_SHIFT, off = a & SIZE_MASK;
+        if (idx >= P->clq_cap_s || off + len > P->clq_cap_i) {
+            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_CLIQUES);
+            return;
+        }
+        for (int i = tid; i < len; i += NT) P->clq_ids[off + i] = ids[i];
+        if (tid == 0) {
+            P->clq_meta[idx] = (off << 24) | (unsigned long long)len;
+            atomicAdd(&P->ctr->cliques, 1ull);
+        }
+        bytes += 4ull * len / NT + 1;
+    }
+
+    // ---------------------------------------------------------------- sort + write
+    // Orders members then candidates by the enumeration order of reference
+    // inc/device_kernels.hpp:31-80 (min(mem) desc, min(can) desc, lvl2adj desc, id desc)
+    // and pushes the child onto the task stack.
+    __device__ void write_child(int cm, const int32_t* R, int nr, int mode) {
+        const int N = cm + nr;
+        // key field widths from this child's maxima
+        uint32_t mmx = 0, mcx = 0, l2x = 0;
+        for (int i = tid; i < N; i += NT) {
+            int x = i < cm ? mem[i] : R[i - cm];
+            gat[i] = x;
+            mmx = max(mmx, min(com[x], cim[x]));
+            mcx = max(mcx, min(coc[x], cic[x]));
+            l2x = max(l2x, pl2[x]);
+        }
+        if (tid == 0) { S->red[2] = 0; S->red[3] = 0; S->red[4] = 0; }
+        __syncthreads();
+        atomicMax((unsigned*)&S->red[2], mmx);
+        atomicMax((unsigned*)&S->red[3], mcx);
+        atomicMax((unsigned*)&S->red[4], l2x);
+        __syncthreads();
+        mmx = (uint32_t)S->red[2]; mcx = (uint32_t)S->red[3]; l2x = (uint32_t)S->red[4];
+        __syncthreads();
+        const uint32_t idx_max = (uint32_t)(P->n - 1);
+        const int bi = bitw(idx_max), bl = bitw(l2x), bc = bitw(mcx), bm = bitw(mmx);
+        const int total = 1 + bm + bc + bl + bi;
+        unsigned long long a = 0;
+        if (tid == 0) {
+            const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)N;
+            a = atomicAdd(&P->ctr->sp_alloc, inc);
+            if ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + N > P->sp_cap_v) {
+                atomicOr(&P->ctr->error, (unsigned)ERR_SPILL);
+                a = ~0ull;
+            }
+        }
+        a = bcast0(a);
+        if (a == ~0ull) return;
+        const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
+        TaskDesc* dd = P->sp_desc;
+        dqc::TaskVertex* dv = P->sp_vtx + o;
+        if (total <= 64 && N <= SORT_N) {
+            int Np = 1;
+            while (Np < N) Np <<= 1;
+            for (int i = tid; i < Np; i += NT) {
+                unsigned long long key = ~0ull;
+                if (i < N) {
+                    int x = gat[i];
+                    unsigned long long cls = i < cm ? 0ull : 1ull;
+                    unsigned long long mm = mmx - min(com[x], cim[x]);
+                    unsigned long long mc = mcx - min(coc[x], cic[x]);
+                    unsigned long long l2 = l2x - pl2[x];
+                    unsigned long long id = idx_max - (uint32_t)x;
+                    key = cls;
+                    key = (key << bm) | mm;
+                    key = (key << bc) | mc;
+                    key = (key << bl) | l2;
+                    key = (key << bi) | id;
+                }
+                S->sortk[i] = key;
+            }
+            __syncthreads();
+            for (int k = 2; k <= Np; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < Np; i += NT) {
+                        int ixj = i ^ j;
+                        if (ixj > i) {
+                            unsigned long long x = S->sortk[i], y = S->sortk[ixj];
+                            bool up = (i & k) == 0;
+                            if ((x > y) == up) { S->sortk[i] = y; S->sortk[ixj] = x; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            const unsigned long long idmask = bi == 0 ? 0ull : ((1ull << bi) - 1);
+            for (int i = tid; i < N; i += NT) {
+                int x = (int)(idx_max - (uint32_t)(S->sortk[i] & idmask));
+                dv[i] = dqc::TaskVertex{(uint32_t)x, com[x], coc[x], cim[x], cic[x], pl2[x]};
+            }
+        } else {
+            // fallback: rank by full comparison (rare: very large children)
+            if (tid == 0) atomicAdd(&P->ctr->slow_sorts, 1ull);
+            for (int i = tid; i < N; i += NT) {
+                int x = gat[i];
+                bool cx = i >= cm;
+                uint32_t mx = min(com[x], cim[x]), cx2 = min(coc[x], cic[x]), lx = pl2[x];
+                int rank = 0;
+                for (int j2 = 0; j2 < N; ++j2) {
+                    int y = gat[j2];
+                    bool cy = j2 >= cm;
+                    if (cy != cx) { rank += cy < cx; continue; }
+                    uint32_t my = min(com[y], cim[y]);
+                    if (my != mx) { rank += my > mx; continue; }
+                    uint32_t cyc = min(coc[y], cic[y]);
+                    if (cyc != cx2) { rank += cyc > cx2; continue; }
+                    uint32_t ly = pl2[y];
+                    if (ly != lx) { rank += ly > lx; continue; }
+                    rank += (uint32_t)y > (uint32_t)x;
+                }
+                dv[rank] = dqc::TaskVertex{(uint32_t)x, com[x], coc[x], cim[x], cic[x], pl2[x]};
+            }
+        }
+        if (tid == 0) {
+            TaskDesc d;
+            d.off = o; d.tot = N; d.nm = cm; d.ncov = 0; d.mode = mode;
+            dd[t] = d;
+        }
+        bytes += 24ull * N / NT + 1;
+        __syncthreads();
+    }
+
+    // ---------------------------------------------------------------- branch steps
+    // reference src/device_kernels.cu:400-531 / src/host_functions.cu:1386-1500.
+    // Returns 0 (stop the task), 2 (skip this branch) or 1; a member falling short wins.
+    __device__ int remove_one(int& tot, int& nc, int nm) {
+        --nc;
+        --tot;
+        if (tot < P->min_size) return 0;
+        const uint32_t mo = (uint32_t)gmd(nm + 1, P->mdo), mi = (uint32_t)gmd(nm + 1, P->mdi);
+        const int p = pids[tot], last = pids[tot - 1];
+        if (tid == 0) bcl(S->plive, p);
+        __syncthreads();
+        int fail = 0, skip = 0;
+        for (unsigned long long e = P->oo[p] + tid; e < P->oo[p + 1]; e += NT) {
+            int w = P->on[e];
+            if (bt(S->plive, w)) {
+                uint32_t ic = pic[w] - 1u;
+                pic[w] = ic;
+                if (pim[w] + ic < mi) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
+            }
+        }
+        if (tid == 0) bytes += 4ull * (P->oo[p + 1] - P->oo[p]) + 16;
+        if (__syncthreads_or(fail)) return 0;
+        for (unsigned long long e = P->io[p] + tid; e < P->io[p + 1]; e += NT) {
+            int w = P->in[e];
+            if (bt(S->plive, w)) {
+                uint32_t oc = poc[w] - 1u;
+                poc[w] = oc;
+                if (pom[w] + oc < mo) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
+            }
+        }
+        if (tid == 0) bytes += 4ull * (P->io[p + 1] - P->io[p]) + 16;
+        if (__syncthreads_or(fail)) return 0;
+        const int sk = __syncthreads_or(skip);
+        for (unsigned long long e = P->to[p] + tid; e < P->to[p + 1]; e += NT) {
+            int w = P->tn[e];
+            if (bt(S->plive, w) && !bt(S->pmem, w)) pl2[w] -= 1u;
+        }
+        if (tid == 0) bytes += 4ull * (P->to[p + 1] - P->to[p]) + 16;
+        __syncthreads();
+        return sk ? 2 : 1;
+    }
+
+    // reference src/host_functions.cu:1333-1383 (host: lvl2adj >= tot - 1) and
+    // src/device_kernels.cu:330-397 (device: lvl2adj >= num_cand - 1)
+    __device__ bool lookahead(int tot, int nm, int nc, int mode) {
+        const uint32_t mo = (uint32_t)gmd(tot, P->mdo), mi = (uint32_t)gmd(tot, P->mdi);
+        const uint32_t l2min = mode == MODE_DEVICE ? (uint32_t)(nc - 1) : (uint32_t)(tot - 1);
+        for (int base = 0; base < tot; base += NT) {
+            int i = base + tid, bad = 0;
+            if (i < tot) {
+                int x = pids[i];
+                bad = (pom[x] + poc[x] < mo) || (pim[x] + pic[x] < mi) || (i >= nm && pl2[x] < l2min);
+            }
+            if (__syncthreads_or(bad)) return false;
+        }
+        return true;
+    }
+
+    // One branch (reference src/device_kernels.cu:92-165 / src/host_functions.cu:1046-1133):
+    // add the last candidate v, diameter + degree pruning, critical-vertex pruning, clique
+    // check, child write. Only members and the candidates in L(v) are materialised: all
+    // others are dropped by diameter pruning before any of their values is read.
+    // Returns true if a child was written.
+    __device__ bool branch(int tot, int nm, int mode, bool first_child_host) {
+        const int v = pids[tot - 1];
+        int cm = nm + 1;
+        if (cm + 1 >= HMAX) {  // member histograms sized for fewer members
+            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_MEMBERS);
+            return false;
+        }
+        for (int w = tid; w < kw; w += NT) { S->cmem[w] = 0; S->ccand[w] = 0; S->cL[w] = 0; S->ccrit[w] = 0; }
+        if (tid == 0) { S->cnt[3] = 0; S->cnt[4] = 0; }
+        clear_hist(cm);
+        __syncthreads();
+        // members: the parent's plus v
+        for (int a = tid; a <= nm; a += NT) {
+            int x = a < nm ? pids[a] : v;
+            mem[a] = x;
+            com[x] = pom[x]; coc[x] = poc[x]; cim[x] = pim[x]; cic[x] = pic[x];
+            bs(S->cmem, x);
+        }
+        // candidates within two hops of v (the others fail diameter pruning)
+        for (unsigned long long e = P->to[v] + tid; e < P->to[v + 1]; e += NT) {
+            int w = P->tn[e];
+            if (bt(S->plive, w) && !bt(S->pmem, w) && w != v) {
+                com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
+                bs(S->cL, w);
+                lst[atomicAdd(&S->cnt[3], 1)] = w;
+            }
+        }
+        if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
+        __syncthreads();
+        const int nl = S->cnt[3];
+        // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
+        for (unsigned long long e = P->oo[v] + tid; e < P->oo[v + 1]; e += NT) {
+            int w = P->on[e];
+            if (bt(S->cmem, w) || bt(S->cL, w)) { cim[w] += 1u; cic[w] -= 1u; }
+        }
+        for (unsigned long long e = P->io[v] + tid; e < P->io[v + 1]; e += NT) {
+            int w = P->in[e];
+            if (bt(S->cmem, w) || bt(S->cL, w)) { com[w] += 1u; coc[w] -= 1u; }
+        }
+        if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
+        __syncthreads();
+        // reference src/device_kernels.cu:830-906 (d_diameter_pruning): basic degree test
+        {
+            const uint32_t mo2 = (uint32_t)gmd(nm + 2, P->mdo), mi2 = (uint32_t)gmd(nm + 2, P->mdi);
+            for (int i = tid; i < nl; i += NT) {
+                int w = lst[i];
+                if (com[w] + coc[w] >= mo2 && cim[w] + cic[w] >= mi2) {
+                    bs(S->ccand, w);
+                    la[atomicAdd(&S->cnt[4], 1)] = w;
+                    atomicAdd(&S->ho[com[w]], 1);
+                    atomicAdd(&S->hi[cim[w]], 1);
+                }
+            }
+        }
+        int nr = read_counter(4);
+        int ub = 0, lbd = 0, meo = 0, mei = 0;
+        int32_t* R = la;
+        if (tid == 0) atomicAdd(&P->ctr->branches, 1ull);
+        bool ok = degree_prune(cm, &R, &nr, tot - 1 - nm, ub, lbd, meo, mei, lb, lc);
+        if (!ok) {
+            // reference src/device_kernels.cu:131-132 (always) / src/host_functions.cu:1081
+            if (mode == MODE_DEVICE || P->host_addfail) check_clique(cm);
+            return false;
+        }
+        int s = critical(cm, R, nr, ub, lbd, meo, mei);
+        if (s == 2) return false;
+        check_clique(cm);
+        if (s == 0) return false;
+        if (nr > 0) {
+            write_child(cm, R, nr, first_child_host ? MODE_HOST_L1 : MODE_DEVICE);
+            return true;
+        }
+        return false;
+    }
+
+    // reference src/device_kernels.cu:605-826 + 908-962 (critical vertex pruning,
+    // diameter pruning by the critical set). Returns 2 (critical failure), 0 or 1.
+    // cm / R / nr are updated in place.
+    // On entry R is la or lb (degree_prune alternates those two and drops into lc).
+    __device__ int critical(int& cm, int32_t*& R, int& nr, int& ub, int& lbd, int& meo, int& mei) {
+        const uint32_t tho = (uint32_t)P->mdo[cm + lbd], thi = (uint32_t)P->mdi[cm + lbd];
+        for (int a = 0; a < cm; ++a) {
+            int x = mem[a];
+            if (com[x] + coc[x] == tho && coc[x] > 0u) {
+                for (unsigned long long e = P->oo[x] + tid; e < P->oo[x + 1]; e += NT) {
+                    int w = P->on[e];
+                    if (bt(S->ccand, w)) bs(S->ccrit, w);
+                }
+                if (tid == 0) bytes += 4ull * (P->oo[x + 1] - P->oo[x]) + 16;
+            }
+            if (cim[x] + cic[x] == thi && cic[x] > 0u) {
+                for (unsigned long long e = P->io[x] + tid; e < P->io[x + 1]; e += NT) {
+                    int w = P->in[e];
+                    if (bt(S->ccand, w)) bs(S->ccrit, w);
+                }
+                if (tid == 0) bytes += 4ull * (P->io[x + 1] - P->io[x]) + 16;
+            }
+        }
+        if (tid == 0) { S->cnt[5] = 0; S->cnt[6] = 0; }
+        __syncthreads();
+        int32_t* K = crit;
+        int32_t* rest = (R == la) ? lb : la;
+        for (int i = tid; i < nr; i += NT) {
+            int w = R[i];
+            if (bt(S->ccrit, w)) K[atomicAdd(&S->cnt[5], 1)] = w;
+            else rest[atomicAdd(&S->cnt[6], 1)] = w;
+        }
+        __syncthreads();
+        const int nk = S->cnt[5], nrest = S->cnt[6];
+        __syncthreads();
+        if (nk == 0) return 1;
+        if (cm + nk + 1 >= HMAX) {
+            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_MEMBERS);
+            return 2;
+        }
+        for (int a = tid; a < cm; a += NT) adjc[mem[a]] = 0;
+        for (int i = tid; i < nr; i += NT) adjc[R[i]] = 0;
+        __syncthreads();
+        for (int i = tid; i < nk; i += NT) {
+            int c = K[i];
+            for (unsigned long long e = P->to[c]; e < P->to[c + 1]; ++e) {
+                int w = P->tn[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicAdd(&adjc[w], 1u);
+            }
+            bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
+        }
+        __syncthreads();
+        int bad = 0;
+        for (int a = tid; a < cm; a += NT) if (adjc[mem[a]] != (uint32_t)nk) bad = 1;
+        for (int i = tid; i < nk; i += NT) if ((int)adjc[K[i]] < nk - 1) bad = 1;
+        if (__syncthreads_or(bad)) return 2;
+        for (int i = tid; i < nk; i += NT) {
+            int c = K[i];
+            for (unsigned long long e = P->oo[c]; e < P->oo[c + 1]; ++e) {
+                int w = P->on[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) { atomicAdd(&cim[w], 1u); atomicSub(&cic[w], 1u); }
+            }
+            for (unsigned long long e = P->io[c]; e < P->io[c + 1]; ++e) {
+                int w = P->in[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) { atomicAdd(&com[w], 1u); atomicSub(&coc[w], 1u); }
+            }
+            bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
+        }
+        __syncthreads();
+        for (int i = tid; i < nk; i += NT) {
+            int c = K[i];
+            bs(S->cmem, c);
+            bcl(S->ccand, c);
+            mem[cm + i] = c;
+        }
+        cm += nk;
+        clear_hist(cm);
+        if (tid == 0) S->cnt[7] = 0;
+        __syncthreads();
+        // reference src/device_kernels.cu:908-962: keep candidates two hops from every
+        // critical vertex
+        int32_t* keep = lc;
+        for (int i = tid; i < nrest; i += NT) {
+            int w = rest[i];
+            if (adjc[w] == (uint32_t)nk) {
+                keep[atomicAdd(&S->cnt[7], 1)] = w;
+                atomicAdd(&S->ho[com[w]], 1);
+                atomicAdd(&S->hi[cim[w]], 1);
+            } else {
+                bcl(S->ccand, w);
+            }
+        }
+        int nkeep = read_counter(7);
+        int32_t* old = R;  // no longer needed: becomes the spare list, rest the drop list
+        R = keep;
+        nr = nkeep;
+        bool ok = degree_prune(cm, &R, &nr, nrest, ub, lbd, meo, mei, old, rest);
+        return ok ? 1 : 0;
+    }
+
+    // ---------------------------------------------------------------- one task
+    // reference src/device_kernels.cu:25-173 (device levels) and
+    // src/host_functions.cu:987-1134 (host levels 0 and 1)
+    __device__ void run_task(const TaskDesc& d, const dqc::TaskVertex* rv) {
+        const int T = d.tot, nm = d.nm, mode = d.mode;
+        const int ncov = mode == MODE_DEVICE ? 0 : d.ncov;
+        for (int i = tid; i < T; i += NT) {
+            dqc::TaskVertex r = rv[i];
+            int x = (int)r.id;
+            pids[i] = x;
+            pom[x] = r.out_mem; poc[x] = r.out_can; pim[x] = r.in_mem; pic[x] = r.in_can; pl2[x] = r.lvl2adj;
+            bs(S->plive, x);
+            if (i < nm) bs(S->pmem, x);
+        }
+        bytes += 24ull * T / NT + 1;
+        if (tid == 0) atomicAdd(&P->ctr->tasks, 1ull);
+        __syncthreads();
+        int tot = T, nc = T - nm;
+        const int expansions = nc;
+        bool first_host_child = (mode == MODE_HOST_L0);
+        for (int j = ncov; j < expansions; ++j) {
+            if (j != ncov) {
+                int r = remove_one(tot, nc, nm);
+                if (r == 0) break;
+                if (r == 2) continue;
+            }
+            if (lookahead(tot, nm, nc, mode)) {
+                emit_clique(pids, tot);
+                break;
+            }
+            bool wrote = branch(tot, nm, mode, first_host_child);
+            if (wrote) first_host_child = false;
+        }
+        for (int w = tid; w < kw; w += NT) { S->plive[w] = 0; S->pmem[w] = 0; }
+        __syncthreads();
+    }
+};
+
+__global__ __launch_bounds__(NT) void expand_kernel(KParams P) {
+    __shared__ Shared S;
+    W w;
+    w.P = &P;
+    w.S = &S;
+    w.tid = threadIdx.x;
+    w.kw = (P.n + 31) >> 5;
+    w.bytes = 0;
+    const unsigned long long k = (unsigned long long)P.n;
+    uint32_t* base = P.scratch + (unsigned long long)blockIdx.x * P.stride;
+    w.pom = base; w.poc = base + k; w.pim = base + 2 * k; w.pic = base + 3 * k; w.pl2 = base + 4 * k;
+    w.com = base + 5 * k; w.coc = base + 6 * k; w.cim = base + 7 * k; w.cic = base + 8 * k; w.adjc = base + 9 * k;
+    int32_t* lists = (int32_t*)(base + 10 * k);
+    w.pids = lists; w.mem = lists + k; w.la = lists + 2 * k; w.lb = lists + 3 * k; w.lc = lists + 4 * k;
+    w.lst = lists + 5 * k; w.crit = lists + 6 * k; w.gat = lists + 7 * k;
+    for (int i = threadIdx.x; i < w.kw; i += NT) { S.plive[i] = 0; S.pmem[i] = 0; }
+    if (threadIdx.x == 0) S.acc_bytes = 0;
+    __syncthreads();
+    while (true) {
+        unsigned long long t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(&P.ctr->next_task, 1ull);
+        t = w.bcast0(t);
+        if (t >= P.in_ntask) break;
+        TaskDesc d = P.in_desc[t];
+        w.run_task(d, P.in_vtx + d.off);
+    }
+    atomicAdd(&S.acc_bytes, w.bytes);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&P.ctr->bytes, S.acc_bytes);
+}
+
+// Copies tasks [first, first + m) of the task stack into the batch buffer.
+__global__ void refill_kernel(const TaskDesc* sp_desc, const dqc::TaskVertex* sp_vtx,
+                              unsigned long long first, unsigned long long m,
+                              TaskDesc* dst_desc, dqc::TaskVertex* dst_vtx,
+                              unsigned long long dst_t0, unsigned long long dst_v0) {
+    const unsigned long long v0 = sp_desc[first].off;
+    const unsigned long long vend = sp_desc[first + m - 1].off + (unsigned long long)sp_desc[first + m - 1].tot;
+    const unsigned long long gid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long gsz = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = gid; i < m; i += gsz) {
+        TaskDesc d = sp_desc[first + i];
+        d.off = d.off - v0 + dst_v0;
+        dst_desc[dst_t0 + i] = d;
+    }
+    for (unsigned long long i = gid; i < vend - v0; i += gsz) dst_vtx[dst_v0 + i] = sp_vtx[v0 + i];
+}
+
+}  // namespace
+
+// =====================================================================================
+// Host side
+struct dqc_handle {
+    int device = 0;
+    int workgroups = 0;
+    dqc_config cfg{};
+    std::string err;
+    // graph
+    int32_t n = 0;
+    unsigned long long *d_oo = nullptr, *d_io = nullptr, *d_to = nullptr;
+    int32_t *d_on = nullptr, *d_in = nullptr, *d_tn = nullptr;
+    int32_t *d_mdo = nullptr, *d_mdi = nullptr, *d_ubo = nullptr, *d_ubi = nullptr;
+    int32_t tlen = 0, min_size = 0;
+    // batch buffer (tasks expanded by one launch) and the LIFO task stack
+    TaskDesc* d_bdesc = nullptr;
+    dqc::TaskVertex* d_bvtx = nullptr;
+    TaskDesc* d_sdesc = nullptr;
+    dqc::TaskVertex* d_svtx = nullptr;
+    unsigned long long st_t = 0, st_v = 0;  // stack depth: tasks, records
+    // cliques
+    int32_t* d_clq_ids = nullptr;
+    unsigned long long* d_clq_meta = nullptr;
+    Counters* d_ctr = nullptr;
+    uint32_t* d_scratch = nullptr;
+    unsigned long long stride = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    dqc_stats st{};
+    unsigned long long max_depth = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+bool hip_ok(dqc_handle* h, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    h->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+template <class T>
+bool dev_alloc(dqc_handle* h, T** p, size_t count, const char* what) {
+    void* q = nullptr;
+    if (!hip_ok(h, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)), what)) return false;
+    h->allocs.push_back(q);
+    *p = (T*)q;
+    return true;
+}
+
+void free_graph(dqc_handle* h) {
+    void* gp[] = {h->d_oo, h->d_io, h->d_to, h->d_on, h->d_in, h->d_tn, h->d_scratch};
+    for (void* p : gp)
+        if (p) {
+            hipFree(p);
+            h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), p), h->allocs.end());
+        }
+    h->d_oo = h->d_io = h->d_to = nullptr;
+    h->d_on = h->d_in = h->d_tn = nullptr;
+    h->d_scratch = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
+    if (!out) return DQC_E_ARG;
+    dqc_handle* h = new dqc_handle();
+    if (cfg) h->cfg = *cfg;
+    else h->cfg.host_addfail_check = 1;
+    dqc_config& c = h->cfg;
+    if (!c.pool_vertices) c.pool_vertices = 1ull << 27;
+    if (!c.spill_tasks) c.spill_tasks = 1ull << 25;
+    if (!c.spill_vertices) c.spill_vertices = 1ull << 30;
+    if (!c.clique_sets) c.clique_sets = 1ull << 25;
+    if (!c.clique_ids) c.clique_ids = 1ull << 29;
+    h->device = c.device;
+    *out = h;
+    if (!hip_ok(h, hipSetDevice(h->device), "hipSetDevice")) return DQC_E_HIP;
+    int cus = 0;
+    if (!hip_ok(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device), "device query"))
+        return DQC_E_HIP;
+    int per_cu = 0;
+    if (!hip_ok(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expand_kernel, NT, 0), "occupancy"))
+        return DQC_E_HIP;
+    if (per_cu < 1) per_cu = 1;
+    h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
+    if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
+    if (!hip_ok(h, hipStreamCreate(&h->stream), "hipStreamCreate")) return DQC_E_HIP;
+    if (!hip_ok(h, hipEventCreate(&h->ev0), "hipEventCreate")) return DQC_E_HIP;
+    if (!hip_ok(h, hipEventCreate(&h->ev1), "hipEventCreate")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_bdesc, c.pool_tasks, "batch alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_bvtx, c.pool_vertices, "batch alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_sdesc, c.spill_tasks, "stack alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_svtx, c.spill_vertices, "stack alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_clq_ids, c.clique_ids, "clique alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_clq_meta, c.clique_sets, "clique alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_ctr, 1, "counter alloc")) return DQC_E_HIP;
+    if (!hip_ok(h, hipMemset(h->d_ctr, 0, sizeof(Counters)), "hipMemset")) return DQC_E_HIP;
+    h->st.workgroups = h->workgroups;
+    return DQC_OK;
+}
+
+dqc_status dqc_close(dqc_handle* h) {
+    if (!h) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    for (void* p : h->allocs) hipFree(p);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return DQC_OK;
+}
+
+const char* dqc_last_error(const dqc_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, const int32_t* out_nbr,
+                            const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
+                            const int32_t* two_nbr) {
+    if (!h || n < 0) return DQC_E_ARG;
+    if (n > KMAX) {
+        h->err = "condensed graph has " + std::to_string(n) + " vertices; this build handles at most " +
+                 std::to_string(KMAX);
+        return DQC_E_UNSUPPORTED;
+    }
+    hipSetDevice(h->device);
+    free_graph(h);
+    h->n = n;
+    size_t m_out = out_off[n], m_in = in_off[n], m_two = two_off[n];
+    if (!dev_alloc(h, &h->d_oo, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_io, (size_t)n + 1, "graph alloc") ||
+        !dev_alloc(h, &h->d_to, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_on, m_out, "graph alloc") ||
+        !dev_alloc(h, &h->d_in, m_in, "graph alloc") || !dev_alloc(h, &h->d_tn, m_two, "graph alloc"))
+        return DQC_E_HIP;
+    bool ok = hip_ok(h, hipMemcpy(h->d_oo, out_off, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice), "upload") &&
+              hip_ok(h, hipMemcpy(h->d_io, in_off, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice), "upload") &&
+              hip_ok(h, hipMemcpy(h->d_to, two_off, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice), "upload") &&
+              (m_out == 0 || hip_ok(h, hipMemcpy(h->d_on, out_nbr, sizeof(int32_t) * m_out, hipMemcpyHostToDevice), "upload")) &&
+              (m_in == 0 || hip_ok(h, hipMemcpy(h->d_in, in_nbr, sizeof(int32_t) * m_in, hipMemcpyHostToDevice), "upload")) &&
+              (m_two == 0 || hip_ok(h, hipMemcpy(h->d_tn, two_nbr, sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload"));
+    if (!ok) return DQC_E_HIP;
+    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each
+    h->stride = 18ull * (unsigned long long)std::max(n, 1);
+    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)h->workgroups, "scratch alloc")) return DQC_E_HIP;
+    h->st.n_condensed = n;
+    return DQC_OK;
+}
+
+dqc_status dqc_set_tables(dqc_handle* h, const int32_t* md_out, const int32_t* md_in, const int32_t* ub_out,
+                          const int32_t* ub_in, int32_t len, int32_t min_size) {
+    if (!h || len < 2 || min_size < 2 || min_size >= len) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    if (h->d_mdo) {
+        for (void* p : {(void*)h->d_mdo, (void*)h->d_mdi, (void*)h->d_ubo, (void*)h->d_ubi}) {
+            hipFree(p);
+            h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), p), h->allocs.end());
+        }
+    }
+    if (!dev_alloc(h, &h->d_mdo, len, "table alloc") || !dev_alloc(h, &h->d_mdi, len, "table alloc") ||
+        !dev_alloc(h, &h->d_ubo, len, "table alloc") || !dev_alloc(h, &h->d_ubi, len, "table alloc"))
+        return DQC_E_HIP;
+    bool ok = hip_ok(h, hipMemcpy(h->d_mdo, md_out, sizeof(int32_t) * len, hipMemcpyHostToDevice), "tables") &&
+              hip_ok(h, hipMemcpy(h->d_mdi, md_in, sizeof(int32_t) * len, hipMemcpyHostToDevice), "tables") &&
+              hip_ok(h, hipMemcpy(h->d_ubo, ub_out, sizeof(int32_t) * len, hipMemcpyHostToDevice), "tables") &&
+              hip_ok(h, hipMemcpy(h->d_ubi, ub_in, sizeof(int32_t) * len, hipMemcpyHostToDevice), "tables");
+    if (!ok) return DQC_E_HIP;
+    h->tlen = len;
+    h->min_size = min_size;
+    return DQC_OK;
+}
+
+dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t ncov) {
+    if (!h || nvert < 0 || ncov < 0 || ncov > nvert) return DQC_E_ARG;
+    if (nvert == 0) return DQC_OK;
+    if (nvert > h->n) return DQC_E_ARG;
+    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + 1 > h->cfg.spill_tasks ||
+        h->st_v + nvert > h->cfg.spill_vertices) {
+        h->err = "level-0 task exceeds the batch or stack capacity";
+        return DQC_E_CAPACITY;
+    }
+    hipSetDevice(h->device);
+    TaskDesc d;
+    d.off = h->st_v;
+    d.tot = nvert; d.nm = 0; d.ncov = ncov; d.mode = MODE_HOST_L0;
+    if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
+        !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
+        return DQC_E_HIP;
+    h->st_t += 1;
+    h->st_v += nvert;
+    return DQC_OK;
+}
+
+// Pops the newest tasks off the stack into the batch buffer (as many as the batch
+// holds); returns the number popped (0 with an error set when none fits).
+static dqc_status pop_batch(dqc_handle* h, unsigned long long* popped) {
+    unsigned long long m = std::min<unsigned long long>(h->st_t, h->cfg.pool_tasks);
+    std::vector<TaskDesc> tail(m);
+    if (!hip_ok(h, hipMemcpy(tail.data(), h->d_sdesc + (h->st_t - m), sizeof(TaskDesc) * m, hipMemcpyDeviceToHost), "stack read"))
+        return DQC_E_HIP;
+    const unsigned long long vend = h->st_v;
+    size_t skip = 0;  // keep the newest tasks whose records fit the batch buffer
+    while (skip < m && vend - tail[skip].off > h->cfg.pool_vertices) ++skip;
+    if (skip == m) { h->err = "a task does not fit the batch buffer (raise pool_vertices)"; return DQC_E_CAPACITY; }
+    const unsigned long long first = h->st_t - m + skip, cnt = m - skip, v0 = tail[skip].off;
+    refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
+    if (!hip_ok(h, hipGetLastError(), "pop launch")) return DQC_E_HIP;
+    h->st_t = first;
+    h->st_v = v0;
+    *popped = cnt;
+    return DQC_OK;
+}
+
+dqc_status dqc_run(dqc_handle* h, int32_t sched) {
+    if (!h || sched != 0) return DQC_E_ARG;
+    if (!h->d_mdo || !h->d_oo) { h->err = "graph and tables must be set first"; return DQC_E_ARG; }
+    hipSetDevice(h->device);
+    auto t0 = std::chrono::steady_clock::now();
+    Counters c{};
+    if (!hip_ok(h, hipMemcpy(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
+    const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
+    auto last_report = t0;
+    while (h->st_t > 0) {
+        unsigned long long m = 0;
+        dqc_status rc = pop_batch(h, &m);
+        if (rc != DQC_OK) return rc;
+        c.next_task = 0;
+        c.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
+        if (!hip_ok(h, hipMemcpyAsync(h->d_ctr, &c, sizeof c, hipMemcpyHostToDevice, h->stream), "ctr write"))
+            return DQC_E_HIP;
+        KParams P;
+        P.n = h->n;
+        P.oo = h->d_oo; P.io = h->d_io; P.to = h->d_to;
+        P.on = h->d_on; P.in = h->d_in; P.tn = h->d_tn;
+        P.mdo = h->d_mdo; P.mdi = h->d_mdi; P.ubo = h->d_ubo; P.ubi = h->d_ubi;
+        P.min_size = h->min_size;
+        P.host_addfail = h->cfg.host_addfail_check;
+        P.in_desc = h->d_bdesc; P.in_vtx = h->d_bvtx; P.in_ntask = m;
+        P.sp_desc = h->d_sdesc; P.sp_vtx = h->d_svtx;
+        P.sp_cap_t = h->cfg.spill_tasks; P.sp_cap_v = h->cfg.spill_vertices;
+        P.clq_ids = h->d_clq_ids; P.clq_meta = h->d_clq_meta;
+        P.clq_cap_s = h->cfg.clique_sets; P.clq_cap_i = h->cfg.clique_ids;
+        P.ctr = h->d_ctr;
+        P.scratch = h->d_scratch;
+        P.stride = h->stride;
+        const int grid = (int)std::min<unsigned long long>((unsigned long long)h->workgroups, m);
+        hipEventRecord(h->ev0, h->stream);
+        expand_kernel<<<grid, NT, 0, h->stream>>>(P);
+        hipEventRecord(h->ev1, h->stream);
+        if (!hip_ok(h, hipGetLastError(), "expand launch")) return DQC_E_HIP;
+        if (!hip_ok(h, hipMemcpyAsync(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
+            !hip_ok(h, hipStreamSynchronize(h->stream), "expand kernel"))
+            return DQC_E_HIP;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        h->st.expand_kernel_ms += ms;
+        h->st.launches++;
+        h->st.tasks_expanded = (int64_t)c.tasks;
+        h->st.branches = (int64_t)c.branches;
+        h->st.raw_cliques = (int64_t)c.cliques;
+        h->st.bytes_scanned = (int64_t)c.bytes;
+        h->st.slow_sorts = (int64_t)c.slow_sorts;
+        if (c.error) {
+            h->err = (c.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
+                   : (c.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
+                   : "a task has more members than the member histograms hold";
+            return (c.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
+        }
+        h->st_t = c.sp_alloc >> CNT_SHIFT;
+        h->st_v = c.sp_alloc & SIZE_MASK;
+        h->max_depth = std::max(h->max_depth, h->st_t);
+        if (progress) {
+            auto now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double>(now - last_report).count() > 5.0) {
+                last_report = now;
+                std::fprintf(stderr, "dqc: %.1fs launches=%lld tasks=%llu stack=%llu cliques=%llu\n",
+                             std::chrono::duration<double>(now - t0).count(), (long long)h->st.launches,
+                             (unsigned long long)c.tasks, h->st_t, (unsigned long long)c.cliques);
+            }
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return DQC_OK;
+}
+
+dqc_status dqc_fetch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids, int32_t* sizes, int32_t* ids) {
+    if (!h || !count || !total_ids) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    Counters c{};
+    if (!hip_ok(h, hipMemcpy(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
+    unsigned long long ns = c.clq_alloc >> CNT_SHIFT, ni = c.clq_alloc & SIZE_MASK;
+    *count = (int64_t)ns;
+    *total_ids = (int64_t)ni;
+    if (!sizes || !ids || ns == 0) return DQC_OK;
+    std::vector<unsigned long long> meta(ns);
+    std::vector<int32_t> raw(ni);
+    if (!hip_ok(h, hipMemcpy(meta.data(), h->d_clq_meta, sizeof(unsigned long long) * ns, hipMemcpyDeviceToHost), "clique read") ||
+        !hip_ok(h, hipMemcpy(raw.data(), h->d_clq_ids, sizeof(int32_t) * ni, hipMemcpyDeviceToHost), "clique read"))
+        return DQC_E_HIP;
+    // sets are stored where their allocation landed; emit them in allocation order
+    size_t p = 0;
+    for (unsigned long long s = 0; s < ns; ++s) {
+        unsigned long long off = meta[s] >> 24, len = meta[s] & 0xFFFFFFull;
+        sizes[s] = (int32_t)len;
+        std::memcpy(ids + p, raw.data() + off, sizeof(int32_t) * len);
+        p += len;
+    }
+    return DQC_OK;
+}
+
+dqc_status dqc_get_stats(const dqc_handle* h, dqc_stats* st) {
+    if (!h || !st) return DQC_E_ARG;
+    *st = h->st;
+    return DQC_OK;
+}
+
+}  // extern "C"

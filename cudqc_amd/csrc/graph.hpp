@@ -1,0 +1,66 @@
+// Host-side graph structures and preprocessing for the MI355X enumerator.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dqc {
+
+// One vertex of a task record, as stored in HBM (24 bytes, AoS). The label is not
+// stored: in a task, positions [0, nm) are members, [nm, nm + ncov) are the covered
+// vertices of the level-0 task, the rest are candidates (reference inc/common.hpp:64-73
+// keeps an explicit int8 label; every stored task there has labels that follow
+// positions this way).
+struct TaskVertex {
+    uint32_t id;
+    uint32_t out_mem, out_can, in_mem, in_can;
+    uint32_t lvl2adj;
+};
+static_assert(sizeof(TaskVertex) == 24, "task record layout");
+
+struct Csr {
+    std::vector<uint64_t> off;  // n + 1
+    std::vector<int32_t> nbr;
+};
+
+// Mirrors CPU_Graph (reference inc/common.hpp:76-94): out / in / two-hop CSR.
+struct Graph {
+    int32_t n = 0;
+    int64_t m = 0;
+    Csr out, in, two;
+};
+
+// Text adjacency loader (reference src/common.cpp:15-108 semantics). Returns 0, or
+// 1 on I/O failure, 2 on a malformed id (outside [0, n)); err gets a message.
+int load_text(const std::string& path, Graph& g, std::string& err);
+
+// Directed two-hop lists (reference src/common.cpp:111-509 semantics).
+void build_twohop(Graph& g, int threads);
+
+// md[0] = 0, md[k] = ceil(gamma (k-1)) in IEEE double (reference
+// src/host_functions.cu:9-15), for k < len.
+std::vector<int32_t> min_degree_table(double gamma, int len);
+// ub[d] = (int)(d / gamma) in IEEE double, the upper-bound division of
+// reference src/host_functions.cu:2165-2166 and src/device_kernels.cu:1402-1403.
+std::vector<int32_t> bound_div_table(double gamma, int len);
+
+// Level-0 peel, cover pruning and graph condensing (reference
+// src/host_functions.cu:461-934). Replaces g by the condensed graph whose ids are the
+// level-0 enumeration ranks, and returns the level-0 task in enumeration order.
+struct Level0 {
+    std::vector<TaskVertex> task;
+    int32_t ncov = 0;              // covered vertices right after the (zero) members
+    std::vector<int32_t> orig_of;  // input id of each condensed id
+};
+void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int min_size,
+                     Level0& out);
+
+// The maximal-set filter of reference src/Quick_rmnonmax.cpp (RemoveNonMax):
+// reads "<len> ids..." records, writes the maximal distinct sets, each ascending,
+// in lexicographic order. Returns the number written, -1 on I/O error. The output
+// file is written only when the input holds at least one set.
+int64_t remove_non_max(const std::string& in_path, const std::string& out_path);
+// Same filter over in-memory sets (each set is sorted in place).
+std::vector<std::vector<int32_t>> maximal_sets(std::vector<std::vector<int32_t>>& sets);
+
+}  // namespace dqc

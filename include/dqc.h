@@ -1,0 +1,127 @@
+/* C-ABI of the MI355X maximal directed quasi-clique enumerator (libcudqc_amd.so).
+ *
+ * The reference (Mike12041204/cuDQC) has no plugin or FFI layer: its enumeration path
+ * sits behind the CLI in program.cu and an in-process C++ host API. These entry points
+ * are what that host API would bind to; each cites the reference interface it replaces.
+ * Plain pointers and sizes only; every function returns a dqc_status (0 = DQC_OK) and
+ * never calls exit(). A handle is single-threaded and owns all device memory it uses;
+ * no caller pointer is retained after a call returns.
+ */
+#ifndef CUDQC_AMD_DQC_H
+#define CUDQC_AMD_DQC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t dqc_status;
+#define DQC_OK 0
+#define DQC_E_HIP 1      /* HIP runtime error (message via dqc_last_error)        */
+#define DQC_E_CAPACITY 2 /* a device pool or clique buffer is too small           */
+#define DQC_E_ARG 3      /* invalid argument (reference program.cu:27-61 checks)   */
+#define DQC_E_IO 4       /* file could not be read or written                      */
+#define DQC_E_UNSUPPORTED 5 /* input outside what this build handles (e.g. size)  */
+
+typedef struct dqc_handle dqc_handle;
+
+/* Capacities are advisory in the way DS_Sizes.csv is for the reference
+ * (reference src/common.cpp:521-557); 0 selects a default. */
+typedef struct {
+    int32_t device;            /* HIP device ordinal                                  */
+    int32_t workgroups;        /* persistent workgroups; 0 = 3 per CU                 */
+    uint64_t pool_tasks;       /* tasks expanded per launch (batch); 0 = 32 per WG    */
+    uint64_t pool_vertices;    /* 24-byte vertex records the batch buffer holds       */
+    uint64_t spill_tasks;      /* task descriptors the LIFO task stack holds          */
+    uint64_t spill_vertices;   /* vertex records the LIFO task stack holds            */
+    uint64_t clique_sets;      /* raw cliques held on the device                      */
+    uint64_t clique_ids;       /* ids of raw cliques held on the device               */
+    int32_t host_addfail_check;/* 1 (default 4-rank union) / 0 (single rank, see
+                                  reference src/host_functions.cu:1081)                */
+} dqc_config;
+
+typedef struct {
+    int64_t tasks_expanded;    /* tasks whose branch loop ran                         */
+    int64_t branches;          /* add-one-vertex steps                                */
+    int64_t raw_cliques;       /* cliques emitted before the maximality filter        */
+    int64_t maximal_cliques;   /* lines of the maximal result (after dqc_finish)      */
+    int64_t bytes_scanned;     /* CSR + task-record bytes the kernels touched         */
+    int64_t launches;          /* expansion kernel launches                           */
+    int64_t slow_sorts;        /* children ordered by the fallback sort               */
+    double expand_kernel_ms;   /* sum of HIP-event times of expansion launches        */
+    double expand_wall_ms;     /* host wall time of dqc_run                           */
+    int32_t n_condensed;       /* vertices after level-0 condensing                   */
+    int32_t workgroups;
+} dqc_stats;
+
+/* Creates a handle on cfg->device (NULL cfg = defaults). */
+dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out);
+dqc_status dqc_close(dqc_handle* h);
+const char* dqc_last_error(const dqc_handle* h);
+
+/* Uploads a condensed graph: out / in / two-hop CSR with 64-bit offsets.
+ * Replaces the CPU_Graph -> GPU copy of reference src/host_functions.cu:346-367. */
+dqc_status dqc_upload_graph(dqc_handle* h, int32_t n,
+                            const uint64_t* out_off, const int32_t* out_nbr,
+                            const uint64_t* in_off, const int32_t* in_nbr,
+                            const uint64_t* two_off, const int32_t* two_nbr);
+
+/* Minimum-degree tables md[k] = ceil(gamma (k-1)) and bound tables ub[d] = (int)(d/gamma),
+ * len entries each (reference src/host_functions.cu:9-15, :2165-2166). */
+dqc_status dqc_set_tables(dqc_handle* h, const int32_t* md_out, const int32_t* md_in,
+                          const int32_t* ub_out, const int32_t* ub_in, int32_t len,
+                          int32_t min_size);
+
+/* Queues the level-0 task: nvert records of 6 uint32 {id, out_mem, out_can, in_mem,
+ * in_can, lvl2adj} in enumeration order, the first ncov being the covered vertices
+ * (reference src/host_functions.cu:668-682 builds this task). */
+dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t ncov);
+
+/* Expands every queued task to completion on the GPU (replaces the level loop of
+ * reference src/host_functions.cu:92-285, the d_expand_level / d_transfer_buffers /
+ * d_fill_from_buffer kernels and the cuTS MPI stealing). sched: 0 = plain order (batches
+ * popped from the top of the LIFO task stack). */
+dqc_status dqc_run(dqc_handle* h, int32_t sched);
+
+/* Raw cliques found so far: sizes (count entries) and their ids concatenated.
+ * Pass NULL buffers to query *count / *total_ids. */
+dqc_status dqc_fetch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids,
+                             int32_t* sizes, int32_t* ids);
+
+dqc_status dqc_get_stats(const dqc_handle* h, dqc_stats* st);
+
+/* Whole drop-in path of reference program.cu:93-170 for one process: load the text
+ * graph, build two-hop lists, tables, level 0 + condense, GPU expansion, then write
+ * raw_path (DQC-T_<tag> format) and max_path (DQC-R_<tag>, RemoveNonMax output; not
+ * created when there are no cliques). cfg may be NULL. Returns the maximal count in
+ * *max_cliques. */
+dqc_status dqc_search_file(const char* graph_path, double out_gamma, double in_gamma,
+                           int32_t min_size, const char* raw_path, const char* max_path,
+                           const dqc_config* cfg, int64_t* max_cliques, dqc_stats* st);
+
+/* Host preprocessing, no GPU needed. dqc_load_graph: text loader (+ two-hop lists when
+ * with_twohop) with the semantics of CPU_Graph::CPU_Graph / GenLevel2NBs (reference
+ * src/common.cpp:15-509); arrays are malloc'd, release them with dqc_free. */
+dqc_status dqc_load_graph(const char* graph_path, int32_t with_twohop, int32_t* n, int64_t* m,
+                          uint64_t** out_off, int32_t** out_nbr, uint64_t** in_off, int32_t** in_nbr,
+                          uint64_t** two_off, int32_t** two_nbr);
+/* Level-0 peel / cover / condense (reference src/host_functions.cu:461-934): the level-0
+ * task (6 uint32 per vertex, enumeration order), its covered count and the input id of
+ * every condensed id. Arrays are malloc'd. */
+dqc_status dqc_level0(const char* graph_path, double out_gamma, double in_gamma, int32_t min_size,
+                      int32_t* n_condensed, int32_t* ncov, uint32_t** records, int32_t** orig_of);
+void dqc_free(void* p);
+
+/* RemoveNonMax (reference inc/Quick_rmnonmax.h:6): returns the number of maximal sets
+ * written, 0 when the input holds none (no output file then), <0 on I/O error. */
+int64_t dqc_remove_non_max(const char* in_path, const char* out_path);
+
+/* Seeded synthetic workloads 1..5 in the reference text format; graph_index selects
+ * one of the 256 graphs of workload 3. Returns the vertex count or <0. */
+int32_t dqc_generate(int32_t config, int32_t graph_index, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUDQC_AMD_DQC_H */

@@ -1,0 +1,111 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+ORACLE — test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never by the
+product package.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                "n_input m_input n_condensed m_condensed twohop_len tasks branches raw_cliques max_cliques children bytes".split()] + \
+               [(n, ctypes.c_double) for n in "t_load t_twohop t_level0 t_expand t_rmnonmax".split()]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.orc_run.restype = ctypes.c_int64
+    lib.orc_run.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(OrcStats)]
+    lib.orc_graph.restype = ctypes.c_int
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    P32 = ctypes.POINTER(ctypes.c_int)
+    lib.orc_graph.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P32, P64,
+                              ctypes.POINTER(P64), ctypes.POINTER(P32), ctypes.POINTER(P64), ctypes.POINTER(P32),
+                              ctypes.POINTER(P64), ctypes.POINTER(P32)]
+    lib.orc_condense_map.restype = ctypes.c_int
+    lib.orc_condense_map.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(P32)]
+    lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_mindeg.argtypes = [ctypes.c_double, ctypes.c_int, P32]
+    lib.orc_rmnonmax.restype = ctypes.c_int
+    lib.orc_rmnonmax.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    _lib = lib
+    return lib
+
+
+def _b(p):
+    return None if p is None else os.fsencode(p)
+
+
+def run(graph_path, out_gamma, in_gamma, min_size, raw_path=None, max_path=None, threads=0,
+        host_addfail_check=True, time_cap_s=0.0):
+    """Whole CPU path; returns (maximal count or -3 when capped, OrcStats)."""
+    st = OrcStats()
+    r = load().orc_run(_b(graph_path), out_gamma, in_gamma, min_size, threads, int(bool(host_addfail_check)),
+                       _b(raw_path), _b(max_path), float(time_cap_s), ctypes.byref(st))
+    return r, st
+
+
+def graph(path, with_twohop=True, threads=1):
+    """Loader (+ two-hop) as python lists of CSR arrays: dict(n, m, out, in, two)."""
+    lib = load()
+    n = ctypes.c_int()
+    m = ctypes.c_int64()
+    ptrs = [ctypes.POINTER(ctypes.c_int64)() if i % 2 == 0 else ctypes.POINTER(ctypes.c_int)() for i in range(6)]
+    rc = lib.orc_graph(_b(path), int(with_twohop), threads, ctypes.byref(n), ctypes.byref(m),
+                       *[ctypes.byref(p) for p in ptrs])
+    if rc != 0:
+        raise RuntimeError(f"orc_graph failed: {rc}")
+    N = n.value
+    res = {"n": N, "m": m.value}
+    names = ["out", "in", "two"]
+    for k, name in enumerate(names):
+        off = [ptrs[2 * k][i] for i in range(N + 1)]
+        nb = [ptrs[2 * k + 1][i] for i in range(off[-1])] if N else []
+        res[name] = [nb[off[i]:off[i + 1]] for i in range(N)]
+    for p in ptrs:
+        lib.orc_free(p)
+    return res
+
+
+def condense_map(path, out_gamma, in_gamma, min_size):
+    lib = load()
+    p = ctypes.POINTER(ctypes.c_int)()
+    k = lib.orc_condense_map(_b(path), out_gamma, in_gamma, min_size, 1, ctypes.byref(p))
+    if k < 0:
+        raise RuntimeError(f"orc_condense_map failed: {k}")
+    res = [p[i] for i in range(k)]
+    lib.orc_free(p)
+    return res
+
+
+def mindeg(gamma, length):
+    arr = (ctypes.c_int * length)()
+    load().orc_mindeg(gamma, length, arr)
+    return list(arr)
+
+
+def rmnonmax(in_path, out_path):
+    return load().orc_rmnonmax(_b(in_path), _b(out_path))

@@ -1,0 +1,82 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle's golden outputs.
+
+Bit-exact bar: the maximal file (DQC-R bytes) must equal the oracle's, and so must the
+scheduling-invariant counts (tasks expanded, branches, raw cliques), which pins the
+shape of the enumeration tree, not just the final set.
+"""
+import hashlib
+import os
+import subprocess
+
+import pytest
+
+from tests.conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def run_case(dqc, c, tmp_path, config=None):
+    mx = str(tmp_path / (c["name"] + ".max"))
+    cnt, st = dqc.search_file(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"],
+                              None, mx, config=config)
+    return cnt, st, mx
+
+
+def test_golden_cases_bit_exact(dqc, manifest, tmp_path):
+    bad = []
+    for c in manifest["cases"]:
+        print("case", c["name"], flush=True)
+        cnt, st, mx = run_case(dqc, c, tmp_path)
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
+                or st.raw_cliques != c["raw_cliques"]):
+            bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"]))
+    assert not bad, bad
+
+
+def test_small_batches_same_result(dqc, manifest, tmp_path):
+    """Scheduling invariance: a tiny batch size (many launches, deep stack) gives the same
+    bytes and counts as the default."""
+    c = next(x for x in manifest["cases"] if x["name"] == "workload3_g0")
+    cfg = dqc.Config()
+    cfg.pool_tasks = 7
+    cfg.workgroups = 5
+    cfg.host_addfail_check = 1
+    cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
+    assert cnt == c["maximal"] and st.tasks_expanded == c["tasks"] and st.branches == c["branches"]
+    assert hashlib.sha256(open(mx, "rb").read()).hexdigest() == c["sha256"]
+    assert st.launches > 100
+
+
+def test_capacity_error_is_clean(dqc, manifest, tmp_path):
+    """A task stack too small for the search fails with DQC_E_CAPACITY, not a fault."""
+    c = next(x for x in manifest["cases"] if x["name"] == "workload3_g2")
+    cfg = dqc.Config()
+    cfg.spill_tasks = 4
+    cfg.spill_vertices = 1 << 16
+    cfg.host_addfail_check = 1
+    with pytest.raises(dqc.DQCError, match="DQC_E_CAPACITY"):
+        run_case(dqc, c, tmp_path, config=cfg)
+
+
+def test_cli_writes_reference_files(dqc, manifest, tmp_path):
+    c = next(x for x in manifest["cases"] if x["name"] == "workload1")
+    ds = tmp_path / "DS_Sizes.csv"
+    ds.write_text("TASKS_SIZE,100000000\n")
+    r = subprocess.run([dqc.CLI_PATH, os.path.join(GOLDEN, c["graph"]), "0.8", "0.8", "10", str(ds), "w1"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "NUMBER OF CLIQUES:   11" in r.stdout
+    assert (tmp_path / "DQC-R_w1").read_bytes() == open(os.path.join(GOLDEN, c["expected"]), "rb").read()
+    raw = (tmp_path / "DQC-T_w1").read_text().splitlines()
+    assert len(raw) == c["raw_cliques"] and all(l.endswith(" ") for l in raw)
+
+
+def test_cli_argument_errors(dqc, tmp_path):
+    r = subprocess.run([dqc.CLI_PATH, "x"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Usage" in r.stdout
+    g = tmp_path / "g.txt"
+    g.write_text("1\n0\n")
+    r = subprocess.run([dqc.CLI_PATH, str(g), "0.3", "0.8", "3", "ds", "t"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "between .5 and 1" in r.stdout
