@@ -18,7 +18,7 @@ STATUS = {0: "DQC_OK", 1: "DQC_E_HIP", 2: "DQC_E_CAPACITY", 3: "DQC_E_ARG", 4: "
 EXPORTS = [
     "dqc_open", "dqc_close", "dqc_last_error", "dqc_upload_graph", "dqc_set_tables", "dqc_push_root",
     "dqc_run", "dqc_fetch_cliques", "dqc_get_stats", "dqc_search_file", "dqc_remove_non_max", "dqc_generate",
-    "dqc_load_graph", "dqc_level0", "dqc_free",
+    "dqc_load_graph", "dqc_level0", "dqc_free", "dqc_step", "dqc_get_phases", "dqc_prepare_file",
 ]
 
 
@@ -76,6 +76,25 @@ def load():
     lib.dqc_level0.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32, P32, P32,
                                ctypes.POINTER(PU32), ctypes.POINTER(P32)]
     lib.dqc_free.argtypes = [ctypes.c_void_p]
+    lib.dqc_open.restype = ctypes.c_int32
+    lib.dqc_open.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]
+    lib.dqc_close.restype = ctypes.c_int32
+    lib.dqc_close.argtypes = [ctypes.c_void_p]
+    lib.dqc_last_error.argtypes = [ctypes.c_void_p]
+    lib.dqc_prepare_file.restype = ctypes.c_int32
+    lib.dqc_prepare_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int32, P32]
+    lib.dqc_step.restype = ctypes.c_int32
+    lib.dqc_step.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, P32]
+    lib.dqc_run.restype = ctypes.c_int32
+    lib.dqc_run.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    lib.dqc_get_stats.restype = ctypes.c_int32
+    lib.dqc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+    lib.dqc_get_phases.restype = ctypes.c_int32
+    lib.dqc_get_phases.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    lib.dqc_fetch_cliques.restype = ctypes.c_int32
+    lib.dqc_fetch_cliques.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                      P32, P32]
     _lib = lib
     return lib
 
@@ -96,6 +115,69 @@ def search_file(graph_path, out_gamma, in_gamma, min_size, raw_path=None, max_pa
     if rc != DQC_OK:
         raise DQCError(f"dqc_search_file failed: {STATUS.get(rc, rc)}")
     return cnt.value, st
+
+
+class Engine:
+    """A device handle (dqc_open) driven step by step: prepare one graph, then expand the
+    task stack in batches. Used by the benchmark to time a bounded number of batches."""
+
+    def __init__(self, config=None):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.dqc_open(ctypes.byref(config) if config is not None else None, ctypes.byref(self.h))
+        self._check(rc, "dqc_open")
+
+    def _check(self, rc, what):
+        if rc != DQC_OK:
+            msg = self.lib.dqc_last_error(self.h) if self.h else b""
+            raise DQCError(f"{what} failed: {STATUS.get(rc, rc)}: {msg.decode(errors='replace') if msg else ''}")
+
+    def prepare(self, graph_path, out_gamma, in_gamma, min_size):
+        k = ctypes.c_int32()
+        self._check(self.lib.dqc_prepare_file(self.h, _b(graph_path), out_gamma, in_gamma, int(min_size),
+                                              ctypes.byref(k)), "dqc_prepare_file")
+        return k.value
+
+    def step(self, launches=1):
+        fin = ctypes.c_int32()
+        self._check(self.lib.dqc_step(self.h, 0, int(launches), ctypes.byref(fin)), "dqc_step")
+        return bool(fin.value)
+
+    def run(self):
+        self._check(self.lib.dqc_run(self.h, 0), "dqc_run")
+
+    def stats(self):
+        st = Stats()
+        self._check(self.lib.dqc_get_stats(self.h, ctypes.byref(st)), "dqc_get_stats")
+        return st
+
+    def phases(self):
+        arr = (ctypes.c_uint64 * 8)()
+        self._check(self.lib.dqc_get_phases(self.h, arr), "dqc_get_phases")
+        return list(arr)
+
+    def cliques(self):
+        cnt, tot = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_cliques(self.h, ctypes.byref(cnt), ctypes.byref(tot), None, None), "fetch")
+        sizes = (ctypes.c_int32 * max(cnt.value, 1))()
+        ids = (ctypes.c_int32 * max(tot.value, 1))()
+        self._check(self.lib.dqc_fetch_cliques(self.h, ctypes.byref(cnt), ctypes.byref(tot), sizes, ids), "fetch")
+        out, p = [], 0
+        for i in range(cnt.value):
+            out.append(list(ids[p:p + sizes[i]]))
+            p += sizes[i]
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.dqc_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def remove_non_max(in_path, out_path):

@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -42,7 +43,7 @@ constexpr int KW = 512;            // bitmap words: condensed ids < 16384
 constexpr int KMAX = KW * 32;
 constexpr int SORT_N = 4096;       // LDS sort capacity (children larger use the fallback)
 constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
-constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2;
+constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2, MODE_SKIP = 3;
 constexpr unsigned long long CNT_SHIFT = 40;  // combined allocation: (count << 40) | size
 constexpr unsigned long long SIZE_MASK = (1ull << CNT_SHIFT) - 1;
 
@@ -53,11 +54,18 @@ struct TaskDesc {
     int32_t tot, nm, ncov, mode;
 };
 
-struct Counters {
-    unsigned long long next_task;
+// Per-queue counters (one per task stack / stream).
+struct QCtr {
+    unsigned long long next_task;  // next work item of the current launch
     unsigned long long sp_alloc;   // task stack top: (tasks << 40) | records
+    unsigned long long root_first; // level-0 split: (branch << 32) | stack index of the first child
+    unsigned long long pad;
+};
+// Counters shared by every launch of a handle.
+struct GCtr {
     unsigned long long clq_alloc;  // clique store: (sets << 40) | ids
     unsigned long long tasks, branches, cliques, bytes, slow_sorts;
+    unsigned long long phase[8];   // realtime ticks per phase (profiling runs)
     unsigned int error;
     unsigned int pad;
 };
@@ -67,7 +75,7 @@ struct KParams {
     const unsigned long long *oo, *io, *to;
     const int32_t *on, *in, *tn;
     const int32_t *mdo, *mdi, *ubo, *ubi;
-    int32_t min_size, host_addfail;
+    int32_t min_size, host_addfail, profile;
     const TaskDesc* in_desc;
     const dqc::TaskVertex* in_vtx;
     unsigned long long in_ntask;
@@ -77,7 +85,12 @@ struct KParams {
     int32_t* clq_ids;
     unsigned long long* clq_meta;  // (start << 24) | len per set
     unsigned long long clq_cap_s, clq_cap_i;
-    Counters* ctr;
+    QCtr* q;
+    GCtr* g;
+    // level-0 split mode: work item t expands branch split_s[t] of the level-0 task
+    const int32_t* split_s;
+    const unsigned long long *lto;  // transposed two-hop CSR (who lists u)
+    const int32_t* ltn;
     uint32_t* scratch;
     unsigned long long stride;  // uint32 words per workgroup
 };
@@ -99,6 +112,20 @@ __device__ __forceinline__ void bcl(uint32_t* bm, int id) { atomicAnd(&bm[id >> 
 
 __device__ __forceinline__ int bitw(unsigned long long x) { return x == 0 ? 0 : 64 - __clzll(x); }
 
+// Number of entries < x in the sorted list a[lo, hi).
+__device__ __forceinline__ uint32_t cnt_lt(const int32_t* a, unsigned long long lo, unsigned long long hi, int x) {
+    unsigned long long l = lo, h = hi;
+    while (l < h) {
+        unsigned long long m = (l + h) >> 1;
+        if (a[m] < x) l = m + 1; else h = m;
+    }
+    return (uint32_t)(l - lo);
+}
+__device__ __forceinline__ bool contains(const int32_t* a, unsigned long long lo, unsigned long long hi, int x) {
+    unsigned long long l = lo + cnt_lt(a, lo, hi, x);
+    return l < hi && a[l] == x;
+}
+
 // All functions below are executed by every thread of the workgroup with uniform
 // control flow; `W` bundles the per-workgroup pointers and uniform scalars.
 struct W {
@@ -109,6 +136,15 @@ struct W {
     uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
     unsigned long long bytes;  // per-thread traffic tally (CSR + records)
+    unsigned long long ph[8];  // thread 0: ticks per phase
+    unsigned long long ph_t;   // thread 0: start of the current phase
+    __device__ void tick(int k) {
+        if (P->profile && tid == 0) {
+            unsigned long long now = __builtin_amdgcn_s_memrealtime();
+            ph[k] += now - ph_t;
+            ph_t = now;
+        }
+    }
 
     __device__ int gmd(int k, const int32_t* md) const { return k < P->min_size ? md[P->min_size] : md[k]; }
 
@@ -171,15 +207,20 @@ struct W {
         return sa[lo] + (i - st[lo]) * lo;
     }
 
-    // smallest i in [a, b] with pred(i), else b + 1 (a > b gives a)
+    // smallest i in [a, b] with pred(i), else b + 1 (a > b gives a). Each round covers
+    // 8 * NT indices; a thread stops at its own first hit (its indices ascend).
     template <class F>
     __device__ int first_true(int a, int b, F pred) {
-        for (int base = a; base <= b; base += NT) {
-            int i = base + tid;
-            int p = (i <= b) && pred(i);
+        for (int base = a; base <= b; base += 8 * NT) {
+            int hit = INT_MAX;
+            for (int u = 0; u < 8; ++u) {
+                int i = base + u * NT + tid;
+                if (i > b) break;
+                if (pred(i)) { hit = i; break; }
+            }
             if (tid == 0) S->red[0] = INT_MAX;
             __syncthreads();
-            if (p) atomicMin(&S->red[0], i);
+            if (hit != INT_MAX) atomicMin(&S->red[0], hit);
             __syncthreads();
             int r = S->red[0];
             __syncthreads();
@@ -187,15 +228,19 @@ struct W {
         }
         return a > b ? a : b + 1;
     }
-    // largest i in [a, b] with pred(i), else a - 1
+    // largest i in [a, b] with pred(i), else a - 1 (rounds of 8 * NT from the top)
     template <class F>
     __device__ int last_true(int a, int b, F pred) {
-        for (int top = b; top >= a; top -= NT) {
-            int i = top - tid;
-            int p = (i >= a) && pred(i);
+        for (int top = b; top >= a; top -= 8 * NT) {
+            int hit = INT_MIN;
+            for (int u = 0; u < 8; ++u) {
+                int i = top - u * NT - tid;
+                if (i < a) break;
+                if (pred(i)) { hit = i; break; }
+            }
             if (tid == 0) S->red[1] = INT_MIN;
             __syncthreads();
-            if (p) atomicMax(&S->red[1], i);
+            if (hit != INT_MIN) atomicMax(&S->red[1], hit);
             __syncthreads();
             int r = S->red[1];
             __syncthreads();
@@ -404,17 +449,17 @@ struct W {
 
     __device__ void emit_clique(const int32_t* ids, int len) {
         unsigned long long a = 0;
-        if (tid == 0) a = atomicAdd(&P->ctr->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
+        if (tid == 0) a = atomicAdd(&P->g->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
         a = bcast0(a);
         unsigned long long idx = a >> CNT_SHIFT, off = a & SIZE_MASK;
         if (idx >= P->clq_cap_s || off + len > P->clq_cap_i) {
-            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_CLIQUES);
+            if (tid == 0) atomicOr(&P->g->error, (unsigned)ERR_CLIQUES);
             return;
         }
         for (int i = tid; i < len; i += NT) P->clq_ids[off + i] = ids[i];
         if (tid == 0) {
             P->clq_meta[idx] = (off << 24) | (unsigned long long)len;
-            atomicAdd(&P->ctr->cliques, 1ull);
+            atomicAdd(&P->g->cliques, 1ull);
         }
         bytes += 4ull * len / NT + 1;
     }
@@ -423,7 +468,7 @@ struct W {
     // Orders members then candidates by the enumeration order of reference
     // inc/device_kernels.hpp:31-80 (min(mem) desc, min(can) desc, lvl2adj desc, id desc)
     // and pushes the child onto the task stack.
-    __device__ void write_child(int cm, const int32_t* R, int nr, int mode) {
+    __device__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
         const int N = cm + nr;
         // key field widths from this child's maxima
         uint32_t mmx = 0, mcx = 0, l2x = 0;
@@ -448,14 +493,14 @@ struct W {
         unsigned long long a = 0;
         if (tid == 0) {
             const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)N;
-            a = atomicAdd(&P->ctr->sp_alloc, inc);
+            a = atomicAdd(&P->q->sp_alloc, inc);
             if ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + N > P->sp_cap_v) {
-                atomicOr(&P->ctr->error, (unsigned)ERR_SPILL);
+                atomicOr(&P->g->error, (unsigned)ERR_SPILL);
                 a = ~0ull;
             }
         }
         a = bcast0(a);
-        if (a == ~0ull) return;
+        if (a == ~0ull) return -1;
         const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
         TaskDesc* dd = P->sp_desc;
         dqc::TaskVertex* dv = P->sp_vtx + o;
@@ -499,7 +544,7 @@ struct W {
             }
         } else {
             // fallback: rank by full comparison (rare: very large children)
-            if (tid == 0) atomicAdd(&P->ctr->slow_sorts, 1ull);
+            if (tid == 0) atomicAdd(&P->g->slow_sorts, 1ull);
             for (int i = tid; i < N; i += NT) {
                 int x = gat[i];
                 bool cx = i >= cm;
@@ -527,6 +572,7 @@ struct W {
         }
         bytes += 24ull * N / NT + 1;
         __syncthreads();
+        return (long long)t;
     }
 
     // ---------------------------------------------------------------- branch steps
@@ -594,15 +640,12 @@ struct W {
     // Returns true if a child was written.
     __device__ bool branch(int tot, int nm, int mode, bool first_child_host) {
         const int v = pids[tot - 1];
-        int cm = nm + 1;
+        const int cm = nm + 1;
         if (cm + 1 >= HMAX) {  // member histograms sized for fewer members
-            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_MEMBERS);
+            if (tid == 0) atomicOr(&P->g->error, (unsigned)ERR_MEMBERS);
             return false;
         }
-        for (int w = tid; w < kw; w += NT) { S->cmem[w] = 0; S->ccand[w] = 0; S->cL[w] = 0; S->ccrit[w] = 0; }
-        if (tid == 0) { S->cnt[3] = 0; S->cnt[4] = 0; }
-        clear_hist(cm);
-        __syncthreads();
+        begin_child(cm);
         // members: the parent's plus v
         for (int a = tid; a <= nm; a += NT) {
             int x = a < nm ? pids[a] : v;
@@ -633,6 +676,58 @@ struct W {
         }
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
+        return branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1) >= 0;
+    }
+
+    __device__ void begin_child(int cm) {
+        for (int w = tid; w < kw; w += NT) { S->cmem[w] = 0; S->ccand[w] = 0; S->cL[w] = 0; S->ccrit[w] = 0; }
+        if (tid == 0) { S->cnt[3] = 0; S->cnt[4] = 0; }
+        clear_hist(cm);
+        __syncthreads();
+    }
+
+    // Branch s of the level-0 task, built without replaying the s earlier removals.
+    // Level-0 positions are condensed ids and nothing there is a member, so after s
+    // removals the live set is [0, k - s), v = k - s - 1, and every value the child
+    // needs is a count on a sorted list: can-degrees count neighbours below v,
+    // lvl2adj loses one per removed vertex (id >= k - s) whose two-hop list holds the
+    // vertex (reference src/host_functions.cu:1386-1500, 1503-1559).
+    __device__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
+        const int v = k - s - 1, tot = k - s;
+        begin_child(1);
+        const unsigned long long *oo = P->oo, *io = P->io;
+        if (tid == 0) {
+            mem[0] = v;
+            com[v] = 0; cim[v] = 0;
+            coc[v] = cnt_lt(P->on, oo[v], oo[v + 1], v);
+            cic[v] = cnt_lt(P->in, io[v], io[v + 1], v);
+            pl2[v] = root[v].lvl2adj - ((uint32_t)(P->lto[v + 1] - P->lto[v]) - cnt_lt(P->ltn, P->lto[v], P->lto[v + 1], tot));
+            bs(S->cmem, v);
+        }
+        for (unsigned long long e = P->to[v] + tid; e < P->to[v + 1]; e += NT) {
+            int u = P->tn[e];
+            if (u >= v) continue;
+            com[u] = contains(P->in, io[v], io[v + 1], u) ? 1u : 0u;   // u -> v
+            cim[u] = contains(P->on, oo[v], oo[v + 1], u) ? 1u : 0u;   // v -> u
+            coc[u] = cnt_lt(P->on, oo[u], oo[u + 1], v);
+            cic[u] = cnt_lt(P->in, io[u], io[u + 1], v);
+            const unsigned long long l0 = P->lto[u], l1 = P->lto[u + 1];
+            pl2[u] = root[u].lvl2adj - ((uint32_t)(l1 - l0) - cnt_lt(P->ltn, l0, l1, tot));
+            bs(S->cL, u);
+            lst[atomicAdd(&S->cnt[3], 1)] = u;
+        }
+        if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
+        __syncthreads();
+        const int nl = S->cnt[3];
+        __syncthreads();
+        return branch_tail(0, tot, nl, MODE_HOST_L0, MODE_DEVICE, s);
+    }
+
+    // Diameter test through child write (reference src/device_kernels.cu:122-165 /
+    // src/host_functions.cu:1067-1133). Members are mem[0..nm] (v last), the L(v)
+    // candidates lst[0..nl) with post-add degrees. Returns the child's stack index or -1.
+    __device__ long long branch_tail(int nm, int tot, int nl, int mode, int child_mode, int split_index) {
+        int cm = nm + 1;
         // reference src/device_kernels.cu:830-906 (d_diameter_pruning): basic degree test
         {
             const uint32_t mo2 = (uint32_t)gmd(nm + 2, P->mdo), mi2 = (uint32_t)gmd(nm + 2, P->mdi);
@@ -649,22 +744,27 @@ struct W {
         int nr = read_counter(4);
         int ub = 0, lbd = 0, meo = 0, mei = 0;
         int32_t* R = la;
-        if (tid == 0) atomicAdd(&P->ctr->branches, 1ull);
+        if (tid == 0) atomicAdd(&P->g->branches, 1ull);
+        tick(3);
         bool ok = degree_prune(cm, &R, &nr, tot - 1 - nm, ub, lbd, meo, mei, lb, lc);
+        tick(4);
         if (!ok) {
             // reference src/device_kernels.cu:131-132 (always) / src/host_functions.cu:1081
             if (mode == MODE_DEVICE || P->host_addfail) check_clique(cm);
-            return false;
+            tick(7);
+            return -1;
         }
         int s = critical(cm, R, nr, ub, lbd, meo, mei);
-        if (s == 2) return false;
+        tick(5);
+        if (s == 2) return -1;
         check_clique(cm);
-        if (s == 0) return false;
-        if (nr > 0) {
-            write_child(cm, R, nr, first_child_host ? MODE_HOST_L1 : MODE_DEVICE);
-            return true;
-        }
-        return false;
+        tick(7);
+        if (s == 0 || nr == 0) return -1;
+        long long t = write_child(cm, R, nr, child_mode);
+        if (split_index >= 0 && t >= 0 && tid == 0)
+            atomicMin(&P->q->root_first, ((unsigned long long)split_index << 32) | (unsigned long long)t);
+        tick(6);
+        return t;
     }
 
     // reference src/device_kernels.cu:605-826 + 908-962 (critical vertex pruning,
@@ -704,7 +804,7 @@ struct W {
         __syncthreads();
         if (nk == 0) return 1;
         if (cm + nk + 1 >= HMAX) {
-            if (tid == 0) atomicOr(&P->ctr->error, (unsigned)ERR_MEMBERS);
+            if (tid == 0) atomicOr(&P->g->error, (unsigned)ERR_MEMBERS);
             return 2;
         }
         for (int a = tid; a < cm; a += NT) adjc[mem[a]] = 0;
@@ -782,22 +882,28 @@ struct W {
             if (i < nm) bs(S->pmem, x);
         }
         bytes += 24ull * T / NT + 1;
-        if (tid == 0) atomicAdd(&P->ctr->tasks, 1ull);
+        if (tid == 0) atomicAdd(&P->g->tasks, 1ull);
         __syncthreads();
         int tot = T, nc = T - nm;
         const int expansions = nc;
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
             if (j != ncov) {
+                tick(0);
                 int r = remove_one(tot, nc, nm);
+                tick(1);
                 if (r == 0) break;
                 if (r == 2) continue;
             }
-            if (lookahead(tot, nm, nc, mode)) {
+            tick(0);
+            bool la = lookahead(tot, nm, nc, mode);
+            tick(2);
+            if (la) {
                 emit_clique(pids, tot);
                 break;
             }
             bool wrote = branch(tot, nm, mode, first_host_child);
+            tick(0);
             if (wrote) first_host_child = false;
         }
         for (int w = tid; w < kw; w += NT) { S->plive[w] = 0; S->pmem[w] = 0; }
@@ -813,6 +919,8 @@ __global__ __launch_bounds__(NT) void expand_kernel(KParams P) {
     w.tid = threadIdx.x;
     w.kw = (P.n + 31) >> 5;
     w.bytes = 0;
+    for (int i = 0; i < 8; ++i) w.ph[i] = 0;
+    w.ph_t = P.profile ? __builtin_amdgcn_s_memrealtime() : 0;
     const unsigned long long k = (unsigned long long)P.n;
     uint32_t* base = P.scratch + (unsigned long long)blockIdx.x * P.stride;
     w.pom = base; w.poc = base + k; w.pim = base + 2 * k; w.pic = base + 3 * k; w.pl2 = base + 4 * k;
@@ -825,15 +933,26 @@ __global__ __launch_bounds__(NT) void expand_kernel(KParams P) {
     __syncthreads();
     while (true) {
         unsigned long long t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(&P.ctr->next_task, 1ull);
+        if (threadIdx.x == 0) t = atomicAdd(&P.q->next_task, 1ull);
         t = w.bcast0(t);
         if (t >= P.in_ntask) break;
-        TaskDesc d = P.in_desc[t];
-        w.run_task(d, P.in_vtx + d.off);
+        if (P.split_s) {
+            const TaskDesc d = P.in_desc[0];
+            w.branch_root(P.split_s[t], d.tot, P.in_vtx + d.off);
+        } else {
+            TaskDesc d = P.in_desc[t];
+            if (d.mode != MODE_SKIP) w.run_task(d, P.in_vtx + d.off);  // lifted to the side stream
+        }
     }
     atomicAdd(&S.acc_bytes, w.bytes);
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&P.ctr->bytes, S.acc_bytes);
+    if (threadIdx.x == 0) {
+        atomicAdd(&P.g->bytes, S.acc_bytes);
+        if (P.profile) {
+            w.tick(0);
+            for (int i = 0; i < 8; ++i) atomicAdd(&P.g->phase[i], w.ph[i]);
+        }
+    }
 }
 
 // Copies tasks [first, first + m) of the task stack into the batch buffer.
@@ -851,6 +970,39 @@ __global__ void refill_kernel(const TaskDesc* sp_desc, const dqc::TaskVertex* sp
         dst_desc[dst_t0 + i] = d;
     }
     for (unsigned long long i = gid; i < vend - v0; i += gsz) dst_vtx[dst_v0 + i] = sp_vtx[v0 + i];
+}
+
+// Per-step outcome of the level-0 branch loop (reference src/host_functions.cu:1014-1044):
+// bit0 = the removal at this step skips the branch (success 2), bit1 = lookahead holds,
+// bit2 = fewer than min_size vertices remain (the loop stops). Uses the same counts as
+// branch_root: after s removals the live set is [0, k - s).
+__global__ void root_flags_kernel(KParams P, const dqc::TaskVertex* root, int k, int ncov, uint8_t* flags) {
+    const int S = k - ncov;
+    const int mo1 = P.mdo[P.min_size], mi1 = P.mdi[P.min_size];  // gmd(nm + 1) with nm = 0
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+        const int tot = k - s;
+        uint8_t f = 0;
+        if (s >= 1) {
+            if (tot < P.min_size) { flags[s] = 4; continue; }
+            const int p = k - s, q = k - s - 1;
+            bool skip = false;
+            if (contains(P.on, P.oo[p], P.oo[p + 1], q) && cnt_lt(P.in, P.io[q], P.io[q + 1], tot) < (uint32_t)mi1) skip = true;
+            if (contains(P.in, P.io[p], P.io[p + 1], q) && cnt_lt(P.on, P.oo[q], P.oo[q + 1], tot) < (uint32_t)mo1) skip = true;
+            if (skip) { flags[s] = 1; continue; }
+        }
+        const uint32_t mo = (uint32_t)(tot < P.min_size ? P.mdo[P.min_size] : P.mdo[tot]);
+        const uint32_t mi = (uint32_t)(tot < P.min_size ? P.mdi[P.min_size] : P.mdi[tot]);
+        const uint32_t l2min = (uint32_t)(tot - 1);
+        bool la = true;
+        for (int i = 0; i < tot && la; ++i) {
+            if (cnt_lt(P.on, P.oo[i], P.oo[i + 1], tot) < mo || cnt_lt(P.in, P.io[i], P.io[i + 1], tot) < mi) { la = false; break; }
+            const unsigned long long l0 = P.lto[i], l1 = P.lto[i + 1];
+            uint32_t l2 = root[i].lvl2adj - ((uint32_t)(l1 - l0) - cnt_lt(P.ltn, l0, l1, tot));
+            if (l2 < l2min) la = false;
+        }
+        if (la) f |= 2;
+        flags[s] = f;
+    }
 }
 
 }  // namespace
@@ -877,13 +1029,32 @@ struct dqc_handle {
     // cliques
     int32_t* d_clq_ids = nullptr;
     unsigned long long* d_clq_meta = nullptr;
-    Counters* d_ctr = nullptr;
+    QCtr* d_q = nullptr;
+    GCtr* d_g = nullptr;
+    unsigned long long* d_lto = nullptr;
+    int32_t* d_ltn = nullptr;
+    uint8_t* d_flags = nullptr;
+    int32_t* d_split = nullptr;
+    std::vector<std::vector<int32_t>> host_cliques;  // level-0 lookahead clique, if any
     uint32_t* d_scratch = nullptr;
     unsigned long long stride = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side stream: the host level-1 task runs there while the main batches continue
+    hipStream_t side = nullptr;
+    QCtr* d_q2 = nullptr;
+    TaskDesc* d_s2desc = nullptr;       // side task stack (children of the lifted task)
+    dqc::TaskVertex* d_s2vtx = nullptr;
+    TaskDesc* d_sbdesc = nullptr;       // side batch buffer (the lifted task)
+    dqc::TaskVertex* d_sbvtx = nullptr;
+    bool side_running = false;
     dqc_stats st{};
     unsigned long long max_depth = 0;
+    unsigned long long last_root_first = ~0ull;
+    int64_t extra_tasks = 0;
+    bool root_pending = false;
+    int profile = 0;
+    unsigned long long phase[8] = {0};
     std::vector<void*> allocs;
 };
 
@@ -905,7 +1076,8 @@ bool dev_alloc(dqc_handle* h, T** p, size_t count, const char* what) {
 }
 
 void free_graph(dqc_handle* h) {
-    void* gp[] = {h->d_oo, h->d_io, h->d_to, h->d_on, h->d_in, h->d_tn, h->d_scratch};
+    void* gp[] = {h->d_oo, h->d_io, h->d_to, h->d_on, h->d_in, h->d_tn, h->d_scratch,
+                  h->d_lto, h->d_ltn, h->d_flags, h->d_split};
     for (void* p : gp)
         if (p) {
             hipFree(p);
@@ -914,6 +1086,7 @@ void free_graph(dqc_handle* h) {
     h->d_oo = h->d_io = h->d_to = nullptr;
     h->d_on = h->d_in = h->d_tn = nullptr;
     h->d_scratch = nullptr;
+    h->d_lto = nullptr; h->d_ltn = nullptr; h->d_flags = nullptr; h->d_split = nullptr;
 }
 
 }  // namespace
@@ -932,6 +1105,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!c.clique_sets) c.clique_sets = 1ull << 25;
     if (!c.clique_ids) c.clique_ids = 1ull << 29;
     h->device = c.device;
+    h->profile = std::getenv("DQC_PHASES") != nullptr;
     *out = h;
     if (!hip_ok(h, hipSetDevice(h->device), "hipSetDevice")) return DQC_E_HIP;
     int cus = 0;
@@ -944,6 +1118,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
     if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
     if (!hip_ok(h, hipStreamCreate(&h->stream), "hipStreamCreate")) return DQC_E_HIP;
+    if (!hip_ok(h, hipStreamCreate(&h->side), "hipStreamCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev0), "hipEventCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev1), "hipEventCreate")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_bdesc, c.pool_tasks, "batch alloc")) return DQC_E_HIP;
@@ -952,8 +1127,16 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!dev_alloc(h, &h->d_svtx, c.spill_vertices, "stack alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_clq_ids, c.clique_ids, "clique alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_clq_meta, c.clique_sets, "clique alloc")) return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_ctr, 1, "counter alloc")) return DQC_E_HIP;
-    if (!hip_ok(h, hipMemset(h->d_ctr, 0, sizeof(Counters)), "hipMemset")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_q, 1, "counter alloc") || !dev_alloc(h, &h->d_g, 1, "counter alloc") ||
+        !dev_alloc(h, &h->d_q2, 1, "counter alloc"))
+        return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_s2desc, c.spill_tasks / 4 + 1, "side alloc") ||
+        !dev_alloc(h, &h->d_s2vtx, c.pool_vertices, "side alloc") || !dev_alloc(h, &h->d_sbdesc, 1, "side alloc") ||
+        !dev_alloc(h, &h->d_sbvtx, c.pool_vertices, "side alloc"))
+        return DQC_E_HIP;
+    if (!hip_ok(h, hipMemset(h->d_q, 0, sizeof(QCtr)), "hipMemset") ||
+        !hip_ok(h, hipMemset(h->d_g, 0, sizeof(GCtr)), "hipMemset"))
+        return DQC_E_HIP;
     h->st.workgroups = h->workgroups;
     return DQC_OK;
 }
@@ -961,10 +1144,13 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
 dqc_status dqc_close(dqc_handle* h) {
     if (!h) return DQC_E_ARG;
     hipSetDevice(h->device);
+    if (h->side) hipStreamSynchronize(h->side);
+    if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) hipFree(p);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
+    if (h->side) hipStreamDestroy(h->side);
     delete h;
     return DQC_OK;
 }
@@ -995,9 +1181,25 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
               (m_in == 0 || hip_ok(h, hipMemcpy(h->d_in, in_nbr, sizeof(int32_t) * m_in, hipMemcpyHostToDevice), "upload")) &&
               (m_two == 0 || hip_ok(h, hipMemcpy(h->d_tn, two_nbr, sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload"));
     if (!ok) return DQC_E_HIP;
+    {
+        // transposed two-hop lists (for each u, the vertices whose list holds u), sorted
+        std::vector<uint64_t> lto((size_t)n + 1, 0);
+        for (size_t e = 0; e < m_two; ++e) lto[two_nbr[e] + 1]++;
+        for (int32_t u = 0; u < n; ++u) lto[u + 1] += lto[u];
+        std::vector<int32_t> ltn(m_two);
+        std::vector<uint64_t> cur(lto.begin(), lto.end() - 1);
+        for (int32_t v = 0; v < n; ++v)
+            for (uint64_t e = two_off[v]; e < two_off[v + 1]; ++e) ltn[cur[two_nbr[e]]++] = v;
+        if (!dev_alloc(h, &h->d_lto, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_ltn, m_two, "graph alloc") ||
+            !dev_alloc(h, &h->d_flags, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_split, (size_t)n + 1, "graph alloc"))
+            return DQC_E_HIP;
+        if (!hip_ok(h, hipMemcpy(h->d_lto, lto.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice), "upload") ||
+            (m_two && !hip_ok(h, hipMemcpy(h->d_ltn, ltn.data(), sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload")))
+            return DQC_E_HIP;
+    }
     // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each
     h->stride = 18ull * (unsigned long long)std::max(n, 1);
-    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)h->workgroups, "scratch alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(h->workgroups + 1), "scratch alloc")) return DQC_E_HIP;
     h->st.n_condensed = n;
     return DQC_OK;
 }
@@ -1043,6 +1245,7 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
         return DQC_E_HIP;
     h->st_t += 1;
     h->st_v += nvert;
+    h->root_pending = std::getenv("DQC_SERIAL_ROOT") == nullptr;
     return DQC_OK;
 }
 
@@ -1066,98 +1269,274 @@ static dqc_status pop_batch(dqc_handle* h, unsigned long long* popped) {
     return DQC_OK;
 }
 
+static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launches, int32_t* finished);
+
 dqc_status dqc_run(dqc_handle* h, int32_t sched) {
+    int32_t fin = 0;
+    return run_launches(h, sched, -1, &fin);
+}
+
+dqc_status dqc_step(dqc_handle* h, int32_t sched, int32_t launches, int32_t* finished) {
+    if (!finished || launches < 1) return DQC_E_ARG;
+    return run_launches(h, sched, launches, finished);
+}
+
+static KParams make_params(dqc_handle* h, unsigned long long ntask) {
+    KParams P;
+    P.n = h->n;
+    P.oo = h->d_oo; P.io = h->d_io; P.to = h->d_to;
+    P.on = h->d_on; P.in = h->d_in; P.tn = h->d_tn;
+    P.mdo = h->d_mdo; P.mdi = h->d_mdi; P.ubo = h->d_ubo; P.ubi = h->d_ubi;
+    P.min_size = h->min_size;
+    P.host_addfail = h->cfg.host_addfail_check;
+    P.profile = h->profile;
+    P.in_desc = h->d_bdesc; P.in_vtx = h->d_bvtx; P.in_ntask = ntask;
+    P.sp_desc = h->d_sdesc; P.sp_vtx = h->d_svtx;
+    P.sp_cap_t = h->cfg.spill_tasks; P.sp_cap_v = h->cfg.spill_vertices;
+    P.clq_ids = h->d_clq_ids; P.clq_meta = h->d_clq_meta;
+    P.clq_cap_s = h->cfg.clique_sets; P.clq_cap_i = h->cfg.clique_ids;
+    P.q = h->d_q;
+    P.g = h->d_g;
+    P.split_s = nullptr;
+    P.lto = h->d_lto; P.ltn = h->d_ltn;
+    P.scratch = h->d_scratch;
+    P.stride = h->stride;
+    return P;
+}
+
+// Launches one expansion over `ntask` work items and folds its counters into the stats.
+static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long ntask) {
+    QCtr q{};
+    q.next_task = 0;
+    q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
+    q.root_first = ~0ull;
+    if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
+    const int grid = (int)std::min<unsigned long long>((unsigned long long)h->workgroups, ntask);
+    hipEventRecord(h->ev0, h->stream);
+    expand_kernel<<<grid, NT, 0, h->stream>>>(P);
+    hipEventRecord(h->ev1, h->stream);
+    if (!hip_ok(h, hipGetLastError(), "expand launch")) return DQC_E_HIP;
+    GCtr g{};
+    if (!hip_ok(h, hipMemcpyAsync(&q, h->d_q, sizeof q, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
+        !hip_ok(h, hipMemcpyAsync(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
+        !hip_ok(h, hipStreamSynchronize(h->stream), "expand kernel"))
+        return DQC_E_HIP;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->st.expand_kernel_ms += ms;
+    h->st.launches++;
+    h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
+    h->st.branches = (int64_t)g.branches;
+    h->st.raw_cliques = (int64_t)g.cliques + (int64_t)h->host_cliques.size();
+    h->st.bytes_scanned = (int64_t)g.bytes;
+    h->st.slow_sorts = (int64_t)g.slow_sorts;
+    for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
+    if (g.error) {
+        h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
+               : (g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
+               : "a task has more members than the member histograms hold";
+        return (g.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
+    }
+    h->st_t = q.sp_alloc >> CNT_SHIFT;
+    h->st_v = q.sp_alloc & SIZE_MASK;
+    h->last_root_first = q.root_first;
+    h->max_depth = std::max(h->max_depth, h->st_t);
+    return DQC_OK;
+}
+
+// Lifts task t of the main stack (the host level-1 task) onto the side stream: its
+// descriptor on the main stack is marked skipped, a copy is expanded by one workgroup
+// on the side stream into the side task stack, and merge_side later moves its children
+// onto the main stack. The level-1 branch loop is sequential by construction; this keeps
+// it from holding up the main batches.
+static dqc_status start_side(dqc_handle* h, unsigned long long t) {
+    TaskDesc d;
+    if (!hip_ok(h, hipMemcpy(&d, h->d_sdesc + t, sizeof d, hipMemcpyDeviceToHost), "side read")) return DQC_E_HIP;
+    if ((unsigned long long)d.tot > h->cfg.pool_vertices) return DQC_OK;  // stays on the main stack
+    TaskDesc c = d;
+    c.off = 0;
+    if (!hip_ok(h, hipMemcpy(h->d_sbvtx, h->d_svtx + d.off, sizeof(dqc::TaskVertex) * d.tot, hipMemcpyDeviceToDevice), "side copy") ||
+        !hip_ok(h, hipMemcpy(h->d_sbdesc, &c, sizeof c, hipMemcpyHostToDevice), "side copy"))
+        return DQC_E_HIP;
+    const int32_t skip = MODE_SKIP;
+    if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &skip, sizeof skip, hipMemcpyHostToDevice), "skip mark"))
+        return DQC_E_HIP;
+    QCtr q{};
+    q.root_first = ~0ull;
+    if (!hip_ok(h, hipMemcpy(h->d_q2, &q, sizeof q, hipMemcpyHostToDevice), "side ctr")) return DQC_E_HIP;
+    KParams P = make_params(h, 1);
+    P.in_desc = h->d_sbdesc; P.in_vtx = h->d_sbvtx;
+    P.sp_desc = h->d_s2desc; P.sp_vtx = h->d_s2vtx;
+    P.sp_cap_t = h->cfg.spill_tasks / 4 + 1; P.sp_cap_v = h->cfg.pool_vertices;
+    P.q = h->d_q2;
+    P.scratch = h->d_scratch + h->stride * (unsigned long long)h->workgroups;  // spare slot
+    expand_kernel<<<1, NT, 0, h->side>>>(P);
+    if (!hip_ok(h, hipGetLastError(), "side launch")) return DQC_E_HIP;
+    h->side_running = true;
+    return DQC_OK;
+}
+
+// If the side kernel has finished (or `wait`), moves its children onto the main stack.
+static dqc_status merge_side(dqc_handle* h, bool wait) {
+    if (!h->side_running) return DQC_OK;
+    if (wait) {
+        if (!hip_ok(h, hipStreamSynchronize(h->side), "side kernel")) return DQC_E_HIP;
+    } else {
+        hipError_t e = hipStreamQuery(h->side);
+        if (e == hipErrorNotReady) return DQC_OK;
+        if (!hip_ok(h, e, "side kernel")) return DQC_E_HIP;
+    }
+    h->side_running = false;
+    QCtr q{};
+    GCtr g{};
+    if (!hip_ok(h, hipMemcpy(&q, h->d_q2, sizeof q, hipMemcpyDeviceToHost), "side ctr") ||
+        !hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read"))
+        return DQC_E_HIP;
+    if (g.error) { h->err = "side stream: task stack or clique store full"; return DQC_E_CAPACITY; }
+    const unsigned long long n2 = q.sp_alloc >> CNT_SHIFT, v2 = q.sp_alloc & SIZE_MASK;
+    if (n2 == 0) return DQC_OK;
+    if (h->st_t + n2 > h->cfg.spill_tasks || h->st_v + v2 > h->cfg.spill_vertices) {
+        h->err = "task stack full (raise spill_tasks / spill_vertices)";
+        return DQC_E_CAPACITY;
+    }
+    refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_s2desc, h->d_s2vtx, 0, n2, h->d_sdesc, h->d_svtx, h->st_t, h->st_v);
+    if (!hip_ok(h, hipGetLastError(), "merge launch") || !hip_ok(h, hipStreamSynchronize(h->stream), "merge"))
+        return DQC_E_HIP;
+    h->st_t += n2;
+    h->st_v += v2;
+    h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
+    return DQC_OK;
+}
+
+// The level-0 task expanded branch-parallel across all workgroups (see branch_root):
+// a flags pass finds where the sequential branch loop would stop and which steps skip
+// their branch, then every remaining branch runs as its own work item. The child of the
+// smallest branch index keeps the host level-1 semantics (reference
+// src/host_functions.cu:1110-1133: the first child goes to the CPU level-1 pass).
+static dqc_status split_root(dqc_handle* h) {
+    unsigned long long m = 0;
+    dqc_status rc = pop_batch(h, &m);
+    if (rc != DQC_OK) return rc;
+    TaskDesc d;
+    if (!hip_ok(h, hipMemcpy(&d, h->d_bdesc, sizeof d, hipMemcpyDeviceToHost), "root read")) return DQC_E_HIP;
+    const int k = d.tot, ncov = d.ncov, S = k - ncov;
+    KParams P = make_params(h, 0);
+    if (S > 0) {
+        root_flags_kernel<<<(S + 255) / 256, 256, 0, h->stream>>>(P, h->d_bvtx + d.off, k, ncov, h->d_flags);
+        if (!hip_ok(h, hipGetLastError(), "flags launch") || !hip_ok(h, hipStreamSynchronize(h->stream), "flags kernel"))
+            return DQC_E_HIP;
+    }
+    std::vector<uint8_t> flags((size_t)std::max(S, 1));
+    if (S > 0 && !hip_ok(h, hipMemcpy(flags.data(), h->d_flags, (size_t)S, hipMemcpyDeviceToHost), "flags read"))
+        return DQC_E_HIP;
+    std::vector<int32_t> work;
+    for (int s = 0; s < S; ++s) {
+        const uint8_t f = flags[s];
+        if (f & 4) break;          // fewer than min_size vertices remain
+        if (f & 1) continue;       // this step's removal skips the branch
+        if (f & 2) {               // lookahead: the whole remaining set is emitted
+            std::vector<int32_t> c((size_t)(k - s));
+            for (int i = 0; i < k - s; ++i) c[i] = i;
+            h->host_cliques.push_back(std::move(c));
+            break;
+        }
+        work.push_back(s);
+    }
+    h->extra_tasks += 1;  // the level-0 task itself
+    if (work.empty()) {
+        h->st.tasks_expanded += 1;
+        h->st.raw_cliques = (int64_t)h->host_cliques.size();
+        return DQC_OK;
+    }
+    if (!hip_ok(h, hipMemcpy(h->d_split, work.data(), sizeof(int32_t) * work.size(), hipMemcpyHostToDevice), "split upload"))
+        return DQC_E_HIP;
+    P.in_ntask = work.size();
+    P.split_s = h->d_split;
+    rc = launch(h, P, work.size());
+    if (rc != DQC_OK) return rc;
+    if (h->last_root_first != ~0ull) {
+        const unsigned long long t = h->last_root_first & 0xFFFFFFFFull;
+        const int32_t mode = MODE_HOST_L1;
+        if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &mode, sizeof mode,
+                                 hipMemcpyHostToDevice), "mode patch"))
+            return DQC_E_HIP;
+        if (std::getenv("DQC_NO_SIDE") == nullptr) return start_side(h, t);
+    }
+    return DQC_OK;
+}
+
+static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launches, int32_t* finished) {
     if (!h || sched != 0) return DQC_E_ARG;
     if (!h->d_mdo || !h->d_oo) { h->err = "graph and tables must be set first"; return DQC_E_ARG; }
     hipSetDevice(h->device);
     auto t0 = std::chrono::steady_clock::now();
-    Counters c{};
-    if (!hip_ok(h, hipMemcpy(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
     const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
     auto last_report = t0;
-    while (h->st_t > 0) {
-        unsigned long long m = 0;
-        dqc_status rc = pop_batch(h, &m);
-        if (rc != DQC_OK) return rc;
-        c.next_task = 0;
-        c.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
-        if (!hip_ok(h, hipMemcpyAsync(h->d_ctr, &c, sizeof c, hipMemcpyHostToDevice, h->stream), "ctr write"))
-            return DQC_E_HIP;
-        KParams P;
-        P.n = h->n;
-        P.oo = h->d_oo; P.io = h->d_io; P.to = h->d_to;
-        P.on = h->d_on; P.in = h->d_in; P.tn = h->d_tn;
-        P.mdo = h->d_mdo; P.mdi = h->d_mdi; P.ubo = h->d_ubo; P.ubi = h->d_ubi;
-        P.min_size = h->min_size;
-        P.host_addfail = h->cfg.host_addfail_check;
-        P.in_desc = h->d_bdesc; P.in_vtx = h->d_bvtx; P.in_ntask = m;
-        P.sp_desc = h->d_sdesc; P.sp_vtx = h->d_svtx;
-        P.sp_cap_t = h->cfg.spill_tasks; P.sp_cap_v = h->cfg.spill_vertices;
-        P.clq_ids = h->d_clq_ids; P.clq_meta = h->d_clq_meta;
-        P.clq_cap_s = h->cfg.clique_sets; P.clq_cap_i = h->cfg.clique_ids;
-        P.ctr = h->d_ctr;
-        P.scratch = h->d_scratch;
-        P.stride = h->stride;
-        const int grid = (int)std::min<unsigned long long>((unsigned long long)h->workgroups, m);
-        hipEventRecord(h->ev0, h->stream);
-        expand_kernel<<<grid, NT, 0, h->stream>>>(P);
-        hipEventRecord(h->ev1, h->stream);
-        if (!hip_ok(h, hipGetLastError(), "expand launch")) return DQC_E_HIP;
-        if (!hip_ok(h, hipMemcpyAsync(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
-            !hip_ok(h, hipStreamSynchronize(h->stream), "expand kernel"))
-            return DQC_E_HIP;
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, h->ev0, h->ev1);
-        h->st.expand_kernel_ms += ms;
-        h->st.launches++;
-        h->st.tasks_expanded = (int64_t)c.tasks;
-        h->st.branches = (int64_t)c.branches;
-        h->st.raw_cliques = (int64_t)c.cliques;
-        h->st.bytes_scanned = (int64_t)c.bytes;
-        h->st.slow_sorts = (int64_t)c.slow_sorts;
-        if (c.error) {
-            h->err = (c.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
-                   : (c.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
-                   : "a task has more members than the member histograms hold";
-            return (c.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
+    int64_t done = 0;
+    while ((h->st_t > 0 || h->side_running) && (max_launches < 0 || done < max_launches)) {
+        dqc_status mrc = merge_side(h, h->st_t == 0);
+        if (mrc != DQC_OK) return mrc;
+        if (h->st_t == 0) continue;
+        ++done;
+        if (h->st_t == 1 && h->root_pending) {
+            h->root_pending = false;
+            dqc_status rc = split_root(h);
+            if (rc != DQC_OK) return rc;
+        } else {
+            unsigned long long m = 0;
+            dqc_status rc = pop_batch(h, &m);
+            if (rc != DQC_OK) return rc;
+            rc = launch(h, make_params(h, m), m);
+            if (rc != DQC_OK) return rc;
         }
-        h->st_t = c.sp_alloc >> CNT_SHIFT;
-        h->st_v = c.sp_alloc & SIZE_MASK;
-        h->max_depth = std::max(h->max_depth, h->st_t);
         if (progress) {
             auto now = std::chrono::steady_clock::now();
             if (std::chrono::duration<double>(now - last_report).count() > 5.0) {
                 last_report = now;
-                std::fprintf(stderr, "dqc: %.1fs launches=%lld tasks=%llu stack=%llu cliques=%llu\n",
+                std::fprintf(stderr, "dqc: %.1fs launches=%lld tasks=%lld stack=%llu cliques=%lld\n",
                              std::chrono::duration<double>(now - t0).count(), (long long)h->st.launches,
-                             (unsigned long long)c.tasks, h->st_t, (unsigned long long)c.cliques);
+                             (long long)h->st.tasks_expanded, h->st_t, (long long)h->st.raw_cliques);
             }
         }
     }
     auto t1 = std::chrono::steady_clock::now();
     h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    *finished = h->st_t == 0 && !h->side_running;
+    return DQC_OK;
+}
+
+dqc_status dqc_get_phases(const dqc_handle* h, uint64_t* ticks8) {
+    if (!h || !ticks8) return DQC_E_ARG;
+    for (int i = 0; i < 8; ++i) ticks8[i] = h->phase[i];
     return DQC_OK;
 }
 
 dqc_status dqc_fetch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids, int32_t* sizes, int32_t* ids) {
     if (!h || !count || !total_ids) return DQC_E_ARG;
     hipSetDevice(h->device);
-    Counters c{};
-    if (!hip_ok(h, hipMemcpy(&c, h->d_ctr, sizeof c, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
-    unsigned long long ns = c.clq_alloc >> CNT_SHIFT, ni = c.clq_alloc & SIZE_MASK;
-    *count = (int64_t)ns;
-    *total_ids = (int64_t)ni;
-    if (!sizes || !ids || ns == 0) return DQC_OK;
+    GCtr g{};
+    if (!hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
+    const unsigned long long ns = g.clq_alloc >> CNT_SHIFT, ni = g.clq_alloc & SIZE_MASK;
+    unsigned long long hs = h->host_cliques.size(), hi = 0;
+    for (auto& c : h->host_cliques) hi += c.size();
+    *count = (int64_t)(ns + hs);
+    *total_ids = (int64_t)(ni + hi);
+    if (!sizes || !ids) return DQC_OK;
+    size_t p = 0, k = 0;
+    for (auto& c : h->host_cliques) {
+        sizes[k++] = (int32_t)c.size();
+        std::memcpy(ids + p, c.data(), sizeof(int32_t) * c.size());
+        p += c.size();
+    }
+    if (ns == 0) return DQC_OK;
     std::vector<unsigned long long> meta(ns);
     std::vector<int32_t> raw(ni);
     if (!hip_ok(h, hipMemcpy(meta.data(), h->d_clq_meta, sizeof(unsigned long long) * ns, hipMemcpyDeviceToHost), "clique read") ||
         !hip_ok(h, hipMemcpy(raw.data(), h->d_clq_ids, sizeof(int32_t) * ni, hipMemcpyDeviceToHost), "clique read"))
         return DQC_E_HIP;
-    // sets are stored where their allocation landed; emit them in allocation order
-    size_t p = 0;
     for (unsigned long long s = 0; s < ns; ++s) {
-        unsigned long long off = meta[s] >> 24, len = meta[s] & 0xFFFFFFull;
-        sizes[s] = (int32_t)len;
+        const unsigned long long off = meta[s] >> 24, len = meta[s] & 0xFFFFFFull;
+        sizes[k++] = (int32_t)len;
         std::memcpy(ids + p, raw.data() + off, sizeof(int32_t) * len);
         p += len;
     }

@@ -81,6 +81,31 @@ dqc_status dqc_level0(const char* graph_path, double out_gamma, double in_gamma,
     return DQC_OK;
 }
 
+dqc_status dqc_prepare_file(dqc_handle* h, const char* graph_path, double out_gamma, double in_gamma,
+                            int32_t min_size, int32_t* n_condensed) {
+    if (!h || !graph_path || !(out_gamma >= 0.5 && out_gamma <= 1.0) || !(in_gamma >= 0.5 && in_gamma <= 1.0) ||
+        min_size <= 1)
+        return DQC_E_ARG;
+    dqc::Graph g;
+    std::string err;
+    int rc = dqc::load_text(graph_path, g, err);
+    if (rc) return rc == 1 ? DQC_E_IO : DQC_E_ARG;
+    dqc::build_twohop(g, omp_get_max_threads());
+    const int len = table_len(g.n, min_size);
+    auto mdo = dqc::min_degree_table(out_gamma, len), mdi = dqc::min_degree_table(in_gamma, len);
+    auto ubo = dqc::bound_div_table(out_gamma, len), ubi = dqc::bound_div_table(in_gamma, len);
+    dqc::Level0 l0;
+    dqc::level0_condense(g, mdo.data(), mdi.data(), min_size, l0);
+    if (n_condensed) *n_condensed = (int32_t)l0.task.size();
+    if (l0.task.empty()) return DQC_OK;
+    dqc_status s = dqc_upload_graph(h, g.n, g.out.off.data(), g.out.nbr.data(), g.in.off.data(), g.in.nbr.data(),
+                                    g.two.off.data(), g.two.nbr.data());
+    if (s != DQC_OK) return s;
+    s = dqc_set_tables(h, mdo.data(), mdi.data(), ubo.data(), ubi.data(), len, min_size);
+    if (s != DQC_OK) return s;
+    return dqc_push_root(h, (const uint32_t*)l0.task.data(), (int32_t)l0.task.size(), l0.ncov);
+}
+
 int64_t dqc_remove_non_max(const char* in_path, const char* out_path) {
     if (!in_path || !out_path) return -1;
     return dqc::remove_non_max(in_path, out_path);

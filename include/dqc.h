@@ -84,6 +84,24 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
  * popped from the top of the LIFO task stack). */
 dqc_status dqc_run(dqc_handle* h, int32_t sched);
 
+/* Runs at most `launches` expansion batches (one batch = the tasks popped from the top
+ * of the task stack and expanded by one kernel launch); *finished = 1 once the stack is
+ * empty. dqc_run is dqc_step until finished. */
+dqc_status dqc_step(dqc_handle* h, int32_t sched, int32_t launches, int32_t* finished);
+
+/* Per-phase realtime-counter ticks summed over workgroups (100 MHz counter), recorded
+ * only when DQC_PHASES is set in the environment at dqc_open: 0 other, 1 remove,
+ * 2 lookahead, 3 child setup + add + diameter, 4 degree pruning, 5 critical, 6 child
+ * sort + write, 7 clique checks. No reference counterpart (profiling aid). */
+dqc_status dqc_get_phases(const dqc_handle* h, uint64_t* ticks8);
+
+/* Host preprocessing + upload for one text graph into an open handle: load, two-hop
+ * lists, tables, level 0 + condense, then queue the level-0 task (reference
+ * program.cu:93-115 and src/host_functions.cu:55-76). Ready for dqc_step / dqc_run.
+ * Optional outputs: condensed vertex count. */
+dqc_status dqc_prepare_file(dqc_handle* h, const char* graph_path, double out_gamma, double in_gamma,
+                            int32_t min_size, int32_t* n_condensed);
+
 /* Raw cliques found so far: sizes (count entries) and their ids concatenated.
  * Pass NULL buffers to query *count / *total_ids. */
 dqc_status dqc_fetch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids,

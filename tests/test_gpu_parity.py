@@ -80,3 +80,15 @@ def test_cli_argument_errors(dqc, tmp_path):
     g.write_text("1\n0\n")
     r = subprocess.run([dqc.CLI_PATH, str(g), "0.3", "0.8", "3", "ds", "t"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "between .5 and 1" in r.stdout
+
+
+def test_serial_level0_same_result(dqc, manifest, tmp_path, monkeypatch):
+    """The level-0 task run by one workgroup (sequential branch loop) and branch-parallel
+    (default) give the same bytes and the same tree counts."""
+    monkeypatch.setenv("DQC_SERIAL_ROOT", "1")
+    for name in ("workload1", "workload3_g0", "workload3_g21"):
+        c = next(x for x in manifest["cases"] if x["name"] == name)
+        cnt, st, mx = run_case(dqc, c, tmp_path)
+        assert cnt == c["maximal"] and st.tasks_expanded == c["tasks"] and st.branches == c["branches"], name
+        if c["sha256"]:
+            assert hashlib.sha256(open(mx, "rb").read()).hexdigest() == c["sha256"], name
