@@ -19,6 +19,7 @@ EXPORTS = [
     "dqc_open", "dqc_close", "dqc_last_error", "dqc_upload_graph", "dqc_set_tables", "dqc_push_root",
     "dqc_run", "dqc_fetch_cliques", "dqc_get_stats", "dqc_search_file", "dqc_remove_non_max", "dqc_generate",
     "dqc_load_graph", "dqc_level0", "dqc_free", "dqc_step", "dqc_get_phases", "dqc_prepare_file",
+    "dqc_push_task", "dqc_fetch_tasks", "dqc_reset",
 ]
 
 
@@ -92,6 +93,12 @@ def load():
     lib.dqc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.dqc_get_phases.restype = ctypes.c_int32
     lib.dqc_get_phases.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    lib.dqc_push_task.restype = ctypes.c_int32
+    lib.dqc_push_task.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    lib.dqc_fetch_tasks.restype = ctypes.c_int32
+    lib.dqc_fetch_tasks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), P32, PU32]
+    lib.dqc_reset.restype = ctypes.c_int32
+    lib.dqc_reset.argtypes = [ctypes.c_void_p]
     lib.dqc_fetch_cliques.restype = ctypes.c_int32
     lib.dqc_fetch_cliques.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                       P32, P32]
@@ -166,6 +173,28 @@ class Engine:
         for i in range(cnt.value):
             out.append(list(ids[p:p + sizes[i]]))
             p += sizes[i]
+        return out
+
+    def reset(self):
+        self._check(self.lib.dqc_reset(self.h), "dqc_reset")
+
+    def push_task(self, records, nm, ncov=0, mode=2):
+        """records: list of 6-tuples (id, out_mem, out_can, in_mem, in_can, lvl2adj)."""
+        flat = (ctypes.c_uint32 * (6 * len(records)))(*[x for r in records for x in r])
+        self._check(self.lib.dqc_push_task(self.h, flat, len(records), nm, ncov, mode), "dqc_push_task")
+
+    def tasks(self):
+        """The task stack as a list of (records, nm, mode)."""
+        nt, nr = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
+        meta = (ctypes.c_int32 * max(4 * nt.value, 1))()
+        rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
+        out = []
+        for i in range(nt.value):
+            off, tot, nm, mode = meta[4 * i:4 * i + 4]
+            recs = [tuple(rec[6 * (off + j):6 * (off + j) + 6]) for j in range(tot)]
+            out.append((recs, nm, mode))
         return out
 
     def close(self):

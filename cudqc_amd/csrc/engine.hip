@@ -99,7 +99,7 @@ struct Shared {
     uint32_t plive[KW], pmem[KW], cmem[KW], ccand[KW], cL[KW], ccrit[KW];
     unsigned long long sortk[SORT_N];
     int32_t ho[HMAX + 2], hi[HMAX + 2], so[HMAX + 2], si[HMAX + 2];
-    int32_t cnt[8];
+    int32_t cnt[12];
     int32_t red[8];
     int32_t lu[12];
     unsigned long long bcast;
@@ -350,18 +350,54 @@ struct W {
         return k;
     }
 
-    // can-degrees of live vertices recounted as |out(x) n R|, |in(x) n R| (R = ccand)
+    // Number of entries of nbr[b, e) whose id is a current candidate (ccand bit), for one
+    // thread, with four independent loads in flight per step.
+    __device__ uint32_t count_cand(const int32_t* nbr, unsigned long long b, unsigned long long e) const {
+        uint32_t c = 0;
+        for (; b + 4 <= e; b += 4) {
+            const int a0 = nbr[b], a1 = nbr[b + 1], a2 = nbr[b + 2], a3 = nbr[b + 3];
+            c += (uint32_t)bt(S->ccand, a0) + (uint32_t)bt(S->ccand, a1) + (uint32_t)bt(S->ccand, a2) +
+                 (uint32_t)bt(S->ccand, a3);
+        }
+        for (; b < e; ++b) c += (uint32_t)bt(S->ccand, nbr[b]);
+        return c;
+    }
+    // Same count for one wavefront (lanes stride the list), summed across the wave.
+    __device__ uint32_t count_cand_wave(const int32_t* nbr, unsigned long long b, unsigned long long e, int lane) const {
+        uint32_t c = 0;
+        for (unsigned long long i = b + lane; i < e; i += 64) c += (uint32_t)bt(S->ccand, nbr[i]);
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        return c;
+    }
+
+    // can-degrees of live vertices recounted as |out(x) n R|, |in(x) n R| (R = ccand).
+    // Vertices with short lists are counted one per thread; vertices with long lists
+    // (hubs) are queued and counted one per wavefront.
     __device__ void recount(int cm, const int32_t* R, int nr) {
+        constexpr unsigned long long LONG_LIST = 192;
         const unsigned long long *oo = P->oo, *io = P->io;
-        for (int i = tid; i < cm + nr; i += NT) {
-            int x = i < cm ? mem[i] : R[i - cm];
-            uint32_t c = 0;
-            for (unsigned long long e = oo[x]; e < oo[x + 1]; ++e) c += bt(S->ccand, P->on[e]);
-            coc[x] = c;
-            c = 0;
-            for (unsigned long long e = io[x]; e < io[x + 1]; ++e) c += bt(S->ccand, P->in[e]);
-            cic[x] = c;
-            bytes += 4ull * ((oo[x + 1] - oo[x]) + (io[x + 1] - io[x])) + 32;
+        const int N = cm + nr;
+        if (tid == 0) S->cnt[8] = 0;
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            const int x = i < cm ? mem[i] : R[i - cm];
+            const unsigned long long ob = oo[x], oe = oo[x + 1], ib = io[x], ie = io[x + 1];
+            bytes += 4ull * ((oe - ob) + (ie - ib)) + 32;
+            if ((oe - ob) + (ie - ib) > LONG_LIST) {
+                gat[atomicAdd(&S->cnt[8], 1)] = x;
+                continue;
+            }
+            coc[x] = count_cand(P->on, ob, oe);
+            cic[x] = count_cand(P->in, ib, ie);
+        }
+        __syncthreads();
+        const int nlong = S->cnt[8];
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int j = wave; j < nlong; j += NT / 64) {
+            const int x = gat[j];
+            const uint32_t co = count_cand_wave(P->on, oo[x], oo[x + 1], lane);
+            const uint32_t ci = count_cand_wave(P->in, io[x], io[x + 1], lane);
+            if (lane == 0) { coc[x] = co; cic[x] = ci; }
         }
         __syncthreads();
     }
@@ -810,32 +846,35 @@ struct W {
         for (int a = tid; a < cm; a += NT) adjc[mem[a]] = 0;
         for (int i = tid; i < nr; i += NT) adjc[R[i]] = 0;
         __syncthreads();
-        for (int i = tid; i < nk; i += NT) {
-            int c = K[i];
-            for (unsigned long long e = P->to[c]; e < P->to[c + 1]; ++e) {
-                int w = P->tn[e];
-                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicAdd(&adjc[w], 1u);
+        // two-hop counts towards the critical set: one critical vertex at a time, its list
+        // spread over the workgroup (entries of one list are distinct, so plain adds)
+        for (int i = 0; i < nk; ++i) {
+            const int c = K[i];
+            for (unsigned long long e = P->to[c] + tid; e < P->to[c + 1]; e += NT) {
+                const int w = P->tn[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) adjc[w] += 1u;
             }
-            bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
+            if (tid == 0) bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
+            __syncthreads();
         }
-        __syncthreads();
         int bad = 0;
         for (int a = tid; a < cm; a += NT) if (adjc[mem[a]] != (uint32_t)nk) bad = 1;
         for (int i = tid; i < nk; i += NT) if ((int)adjc[K[i]] < nk - 1) bad = 1;
         if (__syncthreads_or(bad)) return 2;
-        for (int i = tid; i < nk; i += NT) {
-            int c = K[i];
-            for (unsigned long long e = P->oo[c]; e < P->oo[c + 1]; ++e) {
-                int w = P->on[e];
-                if (bt(S->cmem, w) || bt(S->ccand, w)) { atomicAdd(&cim[w], 1u); atomicSub(&cic[w], 1u); }
+        // the critical vertices join the members (reference src/device_kernels.cu:772-805)
+        for (int i = 0; i < nk; ++i) {
+            const int c = K[i];
+            for (unsigned long long e = P->oo[c] + tid; e < P->oo[c + 1]; e += NT) {
+                const int w = P->on[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) { cim[w] += 1u; cic[w] -= 1u; }
             }
-            for (unsigned long long e = P->io[c]; e < P->io[c + 1]; ++e) {
-                int w = P->in[e];
-                if (bt(S->cmem, w) || bt(S->ccand, w)) { atomicAdd(&com[w], 1u); atomicSub(&coc[w], 1u); }
+            for (unsigned long long e = P->io[c] + tid; e < P->io[c + 1]; e += NT) {
+                const int w = P->in[e];
+                if (bt(S->cmem, w) || bt(S->ccand, w)) { com[w] += 1u; coc[w] -= 1u; }
             }
-            bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
+            if (tid == 0) bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
+            __syncthreads();
         }
-        __syncthreads();
         for (int i = tid; i < nk; i += NT) {
             int c = K[i];
             bs(S->cmem, c);
@@ -1117,8 +1156,9 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (per_cu < 1) per_cu = 1;
     h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
     if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
-    if (!hip_ok(h, hipStreamCreate(&h->stream), "hipStreamCreate")) return DQC_E_HIP;
-    if (!hip_ok(h, hipStreamCreate(&h->side), "hipStreamCreate")) return DQC_E_HIP;
+    // non-blocking: the main loop's synchronous copies must not wait for the side stream
+    if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
+    if (!hip_ok(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev0), "hipEventCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev1), "hipEventCreate")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_bdesc, c.pool_tasks, "batch alloc")) return DQC_E_HIP;
@@ -1249,6 +1289,66 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
     return DQC_OK;
 }
 
+dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t nm, int32_t ncov,
+                         int32_t mode) {
+    if (!h || !records || nvert < 1 || nm < 0 || ncov < 0 || nm + ncov > nvert || mode < 0 || mode > MODE_DEVICE)
+        return DQC_E_ARG;
+    if (nvert > h->n) return DQC_E_ARG;
+    for (int32_t i = 0; i < nvert; ++i)
+        if (records[6 * i] >= (uint32_t)h->n) return DQC_E_ARG;
+    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + 1 > h->cfg.spill_tasks ||
+        h->st_v + nvert > h->cfg.spill_vertices) {
+        h->err = "task exceeds the batch or stack capacity";
+        return DQC_E_CAPACITY;
+    }
+    hipSetDevice(h->device);
+    TaskDesc d;
+    d.off = h->st_v;
+    d.tot = nvert; d.nm = nm; d.ncov = ncov; d.mode = mode;
+    if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
+        !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
+        return DQC_E_HIP;
+    h->st_t += 1;
+    h->st_v += nvert;
+    return DQC_OK;
+}
+
+dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, int32_t* meta, uint32_t* records) {
+    if (!h || !ntasks || !nrecords) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    *ntasks = (int64_t)h->st_t;
+    *nrecords = (int64_t)h->st_v;
+    if (!meta || !records || h->st_t == 0) return DQC_OK;
+    std::vector<TaskDesc> d(h->st_t);
+    if (!hip_ok(h, hipMemcpy(d.data(), h->d_sdesc, sizeof(TaskDesc) * h->st_t, hipMemcpyDeviceToHost), "stack read") ||
+        !hip_ok(h, hipMemcpy(records, h->d_svtx, sizeof(dqc::TaskVertex) * h->st_v, hipMemcpyDeviceToHost), "stack read"))
+        return DQC_E_HIP;
+    for (unsigned long long i = 0; i < h->st_t; ++i) {
+        meta[4 * i] = (int32_t)d[i].off;
+        meta[4 * i + 1] = d[i].tot;
+        meta[4 * i + 2] = d[i].nm;
+        meta[4 * i + 3] = d[i].mode;
+    }
+    return DQC_OK;
+}
+
+dqc_status dqc_reset(dqc_handle* h) {
+    if (!h) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    if (h->side) hipStreamSynchronize(h->side);
+    h->side_running = false;
+    h->st_t = 0;
+    h->st_v = 0;
+    h->root_pending = false;
+    h->host_cliques.clear();
+    GCtr g{};
+    if (!hip_ok(h, hipMemset(h->d_g, 0, sizeof g), "reset") || !hip_ok(h, hipMemset(h->d_q, 0, sizeof(QCtr)), "reset"))
+        return DQC_E_HIP;
+    h->extra_tasks = 0;
+    h->st.tasks_expanded = h->st.branches = h->st.raw_cliques = 0;
+    return DQC_OK;
+}
+
 // Pops the newest tasks off the stack into the batch buffer (as many as the batch
 // holds); returns the number popped (0 with an error set when none fits).
 static dqc_status pop_batch(dqc_handle* h, unsigned long long* popped) {
@@ -1304,6 +1404,15 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     return P;
 }
 
+static void fold_stats(dqc_handle* h, const GCtr& g) {
+    h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
+    h->st.branches = (int64_t)g.branches;
+    h->st.raw_cliques = (int64_t)g.cliques + (int64_t)h->host_cliques.size();
+    h->st.bytes_scanned = (int64_t)g.bytes;
+    h->st.slow_sorts = (int64_t)g.slow_sorts;
+    for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
+}
+
 // Launches one expansion over `ntask` work items and folds its counters into the stats.
 static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long ntask) {
     QCtr q{};
@@ -1325,12 +1434,7 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     hipEventElapsedTime(&ms, h->ev0, h->ev1);
     h->st.expand_kernel_ms += ms;
     h->st.launches++;
-    h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
-    h->st.branches = (int64_t)g.branches;
-    h->st.raw_cliques = (int64_t)g.cliques + (int64_t)h->host_cliques.size();
-    h->st.bytes_scanned = (int64_t)g.bytes;
-    h->st.slow_sorts = (int64_t)g.slow_sorts;
-    for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
+    fold_stats(h, g);
     if (g.error) {
         h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
                : (g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
@@ -1393,6 +1497,7 @@ static dqc_status merge_side(dqc_handle* h, bool wait) {
         !hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read"))
         return DQC_E_HIP;
     if (g.error) { h->err = "side stream: task stack or clique store full"; return DQC_E_CAPACITY; }
+    fold_stats(h, g);
     const unsigned long long n2 = q.sp_alloc >> CNT_SHIFT, v2 = q.sp_alloc & SIZE_MASK;
     if (n2 == 0) return DQC_OK;
     if (h->st_t + n2 > h->cfg.spill_tasks || h->st_v + v2 > h->cfg.spill_vertices) {
@@ -1404,7 +1509,7 @@ static dqc_status merge_side(dqc_handle* h, bool wait) {
         return DQC_E_HIP;
     h->st_t += n2;
     h->st_v += v2;
-    h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
+    fold_stats(h, g);
     return DQC_OK;
 }
 
@@ -1417,6 +1522,8 @@ static dqc_status split_root(dqc_handle* h) {
     unsigned long long m = 0;
     dqc_status rc = pop_batch(h, &m);
     if (rc != DQC_OK) return rc;
+    // the pop runs on the main stream, which does not order against null-stream copies
+    if (!hip_ok(h, hipStreamSynchronize(h->stream), "pop")) return DQC_E_HIP;
     TaskDesc d;
     if (!hip_ok(h, hipMemcpy(&d, h->d_bdesc, sizeof d, hipMemcpyDeviceToHost), "root read")) return DQC_E_HIP;
     const int k = d.tot, ncov = d.ncov, S = k - ncov;
@@ -1498,6 +1605,11 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
                              (long long)h->st.tasks_expanded, h->st_t, (long long)h->st.raw_cliques);
             }
         }
+    }
+    if (h->st_t == 0 && !h->side_running) {  // everything done: final counters
+        GCtr g{};
+        if (!hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
+        fold_stats(h, g);
     }
     auto t1 = std::chrono::steady_clock::now();
     h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -84,6 +84,19 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
  * popped from the top of the LIFO task stack). */
 dqc_status dqc_run(dqc_handle* h, int32_t sched);
 
+/* Queues any task (same record layout as dqc_push_root): nm members first, then ncov
+ * covered vertices, then candidates; mode 0 = host level 0, 1 = host level 1, 2 = device
+ * level (reference src/host_functions.cu:987-1134 vs src/device_kernels.cu:25-173). */
+dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t nm, int32_t ncov,
+                         int32_t mode);
+
+/* Reads the task stack: meta gets 4 int32 per task {record offset, size, members, mode}
+ * and records the 6-uint32 records. NULL buffers query the sizes. */
+dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, int32_t* meta, uint32_t* records);
+
+/* Empties the task stack and the clique store (the graph and tables stay). */
+dqc_status dqc_reset(dqc_handle* h);
+
 /* Runs at most `launches` expansion batches (one batch = the tasks popped from the top
  * of the task stack and expanded by one kernel launch); *finished = 1 once the stack is
  * empty. dqc_run is dqc_step until finished. */

@@ -995,3 +995,91 @@ int orc_rmnonmax(const char* in_path, const char* out_path) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Task-level API: one graph prepared once (load, two-hop, tables, level 0 + condense),
+// then any task expanded one level with the chosen semantics. Records are 6 uint32
+// {id, out_mem, out_can, in_mem, in_can, lvl2adj}; positions [0, nm) are members,
+// [nm, nm + ncov) covered vertices (level 0 only), the rest candidates.
+extern "C" {
+
+struct orc_handle {
+    orc::Graph g;
+    std::vector<int> mdo, mdi;
+    orc::Params P;
+    std::vector<orc::Rec> root;
+    int32_t ncov = 0;
+};
+
+void* orc_open(const char* path, double gout, double gin, int min_size, int threads, int host_addfail_check) {
+    using namespace orc;
+    auto* h = new orc_handle();
+    if (load_text(path, h->g) != 0) { delete h; return nullptr; }
+    build_twohop(h->g, threads < 1 ? 1 : threads);
+    int len = 2 * (std::max(h->g.n, min_size) + 2) + 2;
+    h->mdo = mindeg_table(gout, len);
+    h->mdi = mindeg_table(gin, len);
+    h->P = Params{h->mdo.data(), h->mdi.data(), gout, gin, min_size, host_addfail_check != 0};
+    std::vector<int> orig;
+    level0(h->g, h->P, h->root, orig);
+    while (h->ncov < (int)h->root.size() && h->root[h->ncov].label == 2) ++h->ncov;
+    return h;
+}
+
+void orc_close(void* hv) { delete (orc_handle*)hv; }
+
+// Level-0 task: writes 6 * k uint32 when out != NULL; returns k; *ncov = covered count.
+int orc_root(void* hv, uint32_t* out, int* ncov) {
+    auto* h = (orc_handle*)hv;
+    if (ncov) *ncov = h->ncov;
+    if (out)
+        for (size_t i = 0; i < h->root.size(); ++i) {
+            const orc::Rec& r = h->root[i];
+            uint32_t* o = out + 6 * i;
+            o[0] = r.id; o[1] = r.om; o[2] = r.oc; o[3] = r.im; o[4] = r.ic; o[5] = r.l2;
+        }
+    return (int)h->root.size();
+}
+
+// Expands one task one level. Children: *kid_meta gets 2 int64 per child {record offset,
+// size}, members counted by label; *kids the records; cliques as "len, ids..." int32.
+// All outputs malloc'd (orc_free). Returns the number of children.
+int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int mode, uint32_t** kids,
+               int64_t** kid_meta, int32_t** kid_nm, int32_t** clq, int64_t* clq_len, int64_t* branches) {
+    using namespace orc;
+    auto* h = (orc_handle*)hv;
+    std::vector<Rec> t((size_t)nvert);
+    for (int i = 0; i < nvert; ++i) {
+        const uint32_t* r = rec + 6 * i;
+        t[i].id = r[0]; t[i].om = r[1]; t[i].oc = r[2]; t[i].im = r[3]; t[i].ic = r[4]; t[i].l2 = r[5];
+        t[i].label = i < nm ? 1 : (i < nm + ncov ? 2 : 0);
+    }
+    Sink sink;
+    Expander ex(h->g, h->P);
+    ex.sink = &sink;
+    ex.expand(t.data(), nvert, (Mode)mode);
+    size_t tot = 0;
+    for (auto& k : sink.kids) tot += k.size();
+    *kids = (uint32_t*)std::malloc(sizeof(uint32_t) * 6 * (tot + 1));
+    *kid_meta = (int64_t*)std::malloc(sizeof(int64_t) * 2 * (sink.kids.size() + 1));
+    *kid_nm = (int32_t*)std::malloc(sizeof(int32_t) * (sink.kids.size() + 1));
+    size_t p = 0;
+    for (size_t c = 0; c < sink.kids.size(); ++c) {
+        (*kid_meta)[2 * c] = (int64_t)p;
+        (*kid_meta)[2 * c + 1] = (int64_t)sink.kids[c].size();
+        int m = 0;
+        while (m < (int)sink.kids[c].size() && sink.kids[c][m].label == 1) ++m;
+        (*kid_nm)[c] = m;
+        for (const Rec& r : sink.kids[c]) {
+            uint32_t* o = *kids + 6 * p++;
+            o[0] = r.id; o[1] = r.om; o[2] = r.oc; o[3] = r.im; o[4] = r.ic; o[5] = r.l2;
+        }
+    }
+    *clq = (int32_t*)std::malloc(sizeof(int32_t) * (sink.clq.size() + 1));
+    if (!sink.clq.empty()) std::memcpy(*clq, sink.clq.data(), sizeof(int32_t) * sink.clq.size());
+    *clq_len = (int64_t)sink.clq.size();
+    if (branches) *branches = (int64_t)sink.st.branches;
+    return (int)sink.kids.size();
+}
+
+}  // extern "C"

@@ -51,6 +51,16 @@ def load():
     lib.orc_mindeg.argtypes = [ctypes.c_double, ctypes.c_int, P32]
     lib.orc_rmnonmax.restype = ctypes.c_int
     lib.orc_rmnonmax.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    lib.orc_open.restype = ctypes.c_void_p
+    lib.orc_open.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_close.argtypes = [ctypes.c_void_p]
+    lib.orc_root.restype = ctypes.c_int
+    lib.orc_root.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]
+    PU32 = ctypes.POINTER(ctypes.c_uint32)
+    lib.orc_expand.restype = ctypes.c_int
+    lib.orc_expand.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(PU32), ctypes.POINTER(P64), ctypes.POINTER(P32), ctypes.POINTER(P32),
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     _lib = lib
     return lib
 
@@ -88,6 +98,55 @@ def graph(path, with_twohop=True, threads=1):
     for p in ptrs:
         lib.orc_free(p)
     return res
+
+
+class Tasks:
+    """One graph prepared once; expands given tasks one level (device or host semantics)."""
+
+    HOST_L0, HOST_L1, DEVICE = 0, 1, 2
+
+    def __init__(self, path, out_gamma, in_gamma, min_size, threads=0, host_addfail_check=True):
+        self.lib = load()
+        self.h = self.lib.orc_open(_b(path), out_gamma, in_gamma, min_size, threads or (os.cpu_count() or 1),
+                                   int(bool(host_addfail_check)))
+        if not self.h:
+            raise RuntimeError("orc_open failed")
+
+    def root(self):
+        ncov = ctypes.c_int()
+        k = self.lib.orc_root(self.h, None, ctypes.byref(ncov))
+        arr = (ctypes.c_uint32 * max(6 * k, 1))()
+        self.lib.orc_root(self.h, arr, ctypes.byref(ncov))
+        return [tuple(arr[6 * i:6 * i + 6]) for i in range(k)], ncov.value
+
+    def expand(self, records, nm, ncov=0, mode=2):
+        """Returns (children as (records, nm), cliques as sorted id tuples, branches)."""
+        flat = (ctypes.c_uint32 * max(6 * len(records), 1))(*[x for r in records for x in r])
+        kids = ctypes.POINTER(ctypes.c_uint32)()
+        meta = ctypes.POINTER(ctypes.c_int64)()
+        knm = ctypes.POINTER(ctypes.c_int)()
+        clq = ctypes.POINTER(ctypes.c_int)()
+        clen = ctypes.c_int64()
+        br = ctypes.c_int64()
+        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, ctypes.byref(kids), ctypes.byref(meta),
+                                ctypes.byref(knm), ctypes.byref(clq), ctypes.byref(clen), ctypes.byref(br))
+        out = []
+        for c in range(n):
+            off, size = meta[2 * c], meta[2 * c + 1]
+            out.append(([tuple(kids[6 * (off + j):6 * (off + j) + 6]) for j in range(size)], knm[c]))
+        cl, p = [], 0
+        while p < clen.value:
+            L = clq[p]
+            cl.append(tuple(sorted(clq[p + 1:p + 1 + L])))
+            p += 1 + L
+        for x in (kids, meta, knm, clq):
+            self.lib.orc_free(x)
+        return out, sorted(cl), br.value
+
+    def close(self):
+        if self.h:
+            self.lib.orc_close(self.h)
+            self.h = None
 
 
 def condense_map(path, out_gamma, in_gamma, min_size):
