@@ -16,7 +16,9 @@
 //     LDS bit test;
 //   * HBM arrays indexed by condensed id for the parent's and the child's degrees, so
 //     scatters need no position lookup;
-//   * an LDS bitonic sort of each child on a packed 64-bit enumeration-order key.
+//   * a bitonic sort of each child on a packed 64-bit enumeration-order key, in LDS for
+//     children of up to SORT_N vertices and in the workgroup's HBM area above that (LDS
+//     is kept small so that eight workgroups share a CU).
 //
 // Semantics follow the reference's per-task branch loop exactly (host level 0 / 1 or
 // device levels, selected per task); the functions cite the reference lines they follow.
@@ -41,8 +43,15 @@ namespace {
 constexpr int NT = 256;            // threads per workgroup
 constexpr int KW = 512;            // bitmap words: condensed ids < 16384
 constexpr int KMAX = KW * 32;
-constexpr int SORT_N = 4096;       // LDS sort capacity (children larger use the fallback)
+#ifndef DQC_SORT_N
+#define DQC_SORT_N 256
+#endif
+constexpr int SORT_N = DQC_SORT_N;  // LDS sort capacity (larger children sort in HBM)
 constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
+#ifndef DQC_SCAN_UNR
+#define DQC_SCAN_UNR 8
+#endif
+constexpr int SCAN_UNR = DQC_SCAN_UNR;  // list entries in flight per thread in list scans
 constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2, MODE_SKIP = 3;
 constexpr unsigned long long CNT_SHIFT = 40;  // combined allocation: (count << 40) | size
 constexpr unsigned long long SIZE_MASK = (1ull << CNT_SHIFT) - 1;
@@ -135,10 +144,11 @@ struct W {
     int kw;  // bitmap words in use
     uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
+    unsigned long long* hsort;  // HBM sort keys for children above SORT_N (2k entries)
     unsigned long long bytes;  // per-thread traffic tally (CSR + records)
     unsigned long long ph[8];  // thread 0: ticks per phase
     unsigned long long ph_t;   // thread 0: start of the current phase
-    __device__ void tick(int k) {
+    __device__ __forceinline__ void tick(int k) {
         if (P->profile && tid == 0) {
             unsigned long long now = __builtin_amdgcn_s_memrealtime();
             ph[k] += now - ph_t;
@@ -146,26 +156,62 @@ struct W {
         }
     }
 
-    __device__ int gmd(int k, const int32_t* md) const { return k < P->min_size ? md[P->min_size] : md[k]; }
+    __device__ __forceinline__ int gmd(int k, const int32_t* md) const { return k < P->min_size ? md[P->min_size] : md[k]; }
 
     // one broadcast of a value computed by thread 0 (two barriers: write, then read)
-    __device__ unsigned long long bcast0(unsigned long long v) {
+    __device__ __forceinline__ unsigned long long bcast0(unsigned long long v) {
         if (tid == 0) S->bcast = v;
         __syncthreads();
         unsigned long long r = S->bcast;
         __syncthreads();
         return r;
     }
-    __device__ int read_counter(int slot) {
+    __device__ __forceinline__ int read_counter(int slot) {
         __syncthreads();
         int v = S->cnt[slot];
         __syncthreads();
         return v;
     }
 
+    // Visits every entry of the list a[b, e) once, spread over the workgroup. Each thread
+    // issues SCAN_UNR independent loads before it uses any of them, so a long list costs
+    // one memory round trip per SCAN_UNR * NT entries instead of one per NT.
+    template <class F>
+    __device__ __forceinline__ void wg_scan(const int32_t* __restrict__ a, unsigned long long b,
+                                            unsigned long long e, F&& f) const {
+        unsigned long long i = b + (unsigned long long)tid;
+        for (; i + (SCAN_UNR - 1) * NT < e; i += SCAN_UNR * NT) {
+            int v[SCAN_UNR];
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u) v[u] = a[i + (unsigned long long)u * NT];
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u) f(v[u]);
+        }
+        int v[SCAN_UNR];
+#pragma unroll
+        for (int u = 0; u < SCAN_UNR; ++u) v[u] = i + (unsigned long long)u * NT < e ? a[i + (unsigned long long)u * NT] : -1;
+#pragma unroll
+        for (int u = 0; u < SCAN_UNR; ++u)
+            if (v[u] >= 0) f(v[u]);
+    }
+
+    // Visits a[b, e) with the calling thread alone, 8 loads in flight.
+    template <class F>
+    __device__ __forceinline__ void thread_scan(const int32_t* __restrict__ a, unsigned long long b,
+                                                unsigned long long e, F&& f) const {
+        for (; b + 8 <= e; b += 8) {
+            int v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = a[b + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) f(v[u]);
+        }
+        for (; b < e; ++b) f(a[b]);
+    }
+
     // ---------------------------------------------------------------- predicates
     // reference inc/device_kernels.hpp:111-140 (d_cand_isvalid); unsigned compares kept
-    __device__ bool cand_valid(int w, int cs, int ub, int lb, int meo, int mei) const {
+    __device__ __forceinline__ bool cand_valid(int w, int cs, int ub, int lb, int meo, int mei) const {
         const int32_t *mdo = P->mdo, *mdi = P->mdi;
         uint32_t om = com[w], oc = coc[w], im = cim[w], ic = cic[w];
         if (om + oc < (uint32_t)meo) return false;
@@ -179,7 +225,7 @@ struct W {
         return true;
     }
     // reference inc/device_kernels.hpp:141-178 (d_vert_isextendable)
-    __device__ bool vert_ext(int w, int cs, int ub, int lb, int meo, int mei) const {
+    __device__ __forceinline__ bool vert_ext(int w, int cs, int ub, int lb, int meo, int mei) const {
         const int32_t *mdo = P->mdo, *mdi = P->mdi;
         uint32_t om = com[w], oc = coc[w], im = cim[w], ic = cic[w];
         if (om + oc < (uint32_t)meo) return false;
@@ -198,7 +244,7 @@ struct W {
     // ---------------------------------------------------------------- LU bounds
     // Sum of the i largest candidate member degrees, from the value tables built by
     // lu_tables (start index and sum-above per value, values 0..cm).
-    __device__ int psum(const int32_t* st, const int32_t* sa, int cm, int i) const {
+    __device__ __forceinline__ int psum(const int32_t* st, const int32_t* sa, int cm, int i) const {
         int lo = 0, hi = cm;  // smallest x with st[x] <= i
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
@@ -210,7 +256,7 @@ struct W {
     // smallest i in [a, b] with pred(i), else b + 1 (a > b gives a). Each round covers
     // 8 * NT indices; a thread stops at its own first hit (its indices ascend).
     template <class F>
-    __device__ int first_true(int a, int b, F pred) {
+    __device__ __forceinline__ int first_true(int a, int b, F pred) {
         for (int base = a; base <= b; base += 8 * NT) {
             int hit = INT_MAX;
             for (int u = 0; u < 8; ++u) {
@@ -230,7 +276,7 @@ struct W {
     }
     // largest i in [a, b] with pred(i), else a - 1 (rounds of 8 * NT from the top)
     template <class F>
-    __device__ int last_true(int a, int b, F pred) {
+    __device__ __forceinline__ int last_true(int a, int b, F pred) {
         for (int top = b; top >= a; top -= 8 * NT) {
             int hit = INT_MIN;
             for (int u = 0; u < 8; ++u) {
@@ -252,7 +298,7 @@ struct W {
     // reference src/device_kernels.cu:1307-1486 (d_calculate_LU_bounds), with the
     // member scan on thread 0 and the candidate-prefix loops as parallel searches over
     // the sorted candidate member degrees (held as histograms in S->ho / S->hi).
-    __device__ bool lu_bounds(int cm, int rc, int& ub, int& lb, int& meo, int& mei) {
+    __device__ __forceinline__ bool lu_bounds(int cm, int rc, int& ub, int& lb, int& meo, int& mei) {
         const int32_t *mdo = P->mdo, *mdi = P->mdi;
         if (tid == 0) {
             int x = mem[0];
@@ -321,11 +367,11 @@ struct W {
         return true;
     }
 
-    __device__ void clear_hist(int cm) {
+    __device__ __forceinline__ void clear_hist(int cm) {
         for (int v = tid; v <= cm + 1 && v < HMAX + 2; v += NT) { S->ho[v] = 0; S->hi[v] = 0; }
     }
 
-    __device__ bool members_ext(int cm, int ub, int lb, int meo, int mei) {
+    __device__ __forceinline__ bool members_ext(int cm, int ub, int lb, int meo, int mei) {
         int bad = 0;
         for (int a = tid; a < cm; a += NT)
             if (!vert_ext(mem[a], cm, ub, lb, meo, mei)) bad = 1;
@@ -334,7 +380,7 @@ struct W {
 
     // Splits R[0..nr) by validity: valid ids to keep[], invalid ones to drop[] (their
     // candidate bit cleared). Returns the kept count; *nd gets the dropped count.
-    __device__ int filter(const int32_t* R, int nr, int32_t* keep, int32_t* drop, int* nd,
+    __device__ __forceinline__ int filter(const int32_t* R, int nr, int32_t* keep, int32_t* drop, int* nd,
                           int cm, int ub, int lb, int meo, int mei) {
         if (tid == 0) { S->cnt[0] = 0; S->cnt[1] = 0; }
         __syncthreads();
@@ -352,20 +398,28 @@ struct W {
 
     // Number of entries of nbr[b, e) whose id is a current candidate (ccand bit), for one
     // thread, with four independent loads in flight per step.
-    __device__ uint32_t count_cand(const int32_t* nbr, unsigned long long b, unsigned long long e) const {
+    __device__ __forceinline__ uint32_t count_cand(const int32_t* nbr, unsigned long long b, unsigned long long e) const {
         uint32_t c = 0;
-        for (; b + 4 <= e; b += 4) {
-            const int a0 = nbr[b], a1 = nbr[b + 1], a2 = nbr[b + 2], a3 = nbr[b + 3];
-            c += (uint32_t)bt(S->ccand, a0) + (uint32_t)bt(S->ccand, a1) + (uint32_t)bt(S->ccand, a2) +
-                 (uint32_t)bt(S->ccand, a3);
+        for (; b + 8 <= e; b += 8) {
+            int a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = nbr[b + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c += (uint32_t)bt(S->ccand, a[u]);
         }
         for (; b < e; ++b) c += (uint32_t)bt(S->ccand, nbr[b]);
         return c;
     }
     // Same count for one wavefront (lanes stride the list), summed across the wave.
-    __device__ uint32_t count_cand_wave(const int32_t* nbr, unsigned long long b, unsigned long long e, int lane) const {
+    __device__ __forceinline__ uint32_t count_cand_wave(const int32_t* nbr, unsigned long long b, unsigned long long e, int lane) const {
         uint32_t c = 0;
-        for (unsigned long long i = b + lane; i < e; i += 64) c += (uint32_t)bt(S->ccand, nbr[i]);
+        unsigned long long i = b + lane;
+        for (; i + 3 * 64 < e; i += 4 * 64) {
+            const int a0 = nbr[i], a1 = nbr[i + 64], a2 = nbr[i + 128], a3 = nbr[i + 192];
+            c += (uint32_t)bt(S->ccand, a0) + (uint32_t)bt(S->ccand, a1) + (uint32_t)bt(S->ccand, a2) +
+                 (uint32_t)bt(S->ccand, a3);
+        }
+        for (; i < e; i += 64) c += (uint32_t)bt(S->ccand, nbr[i]);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
         return c;
     }
@@ -373,7 +427,7 @@ struct W {
     // can-degrees of live vertices recounted as |out(x) n R|, |in(x) n R| (R = ccand).
     // Vertices with short lists are counted one per thread; vertices with long lists
     // (hubs) are queued and counted one per wavefront.
-    __device__ void recount(int cm, const int32_t* R, int nr) {
+    __device__ __forceinline__ void recount(int cm, const int32_t* R, int nr) {
         constexpr unsigned long long LONG_LIST = 192;
         const unsigned long long *oo = P->oo, *io = P->io;
         const int N = cm + nr;
@@ -402,24 +456,22 @@ struct W {
         __syncthreads();
     }
     // can-degrees of live vertices decremented by the arcs of dropped candidates
-    __device__ void decrement(const int32_t* D, int nd) {
+    __device__ __forceinline__ void decrement(const int32_t* D, int nd) {
         const unsigned long long *oo = P->oo, *io = P->io;
         for (int i = tid; i < nd; i += NT) {
             int u = D[i];
-            for (unsigned long long e = oo[u]; e < oo[u + 1]; ++e) {
-                int w = P->on[e];
+            thread_scan(P->on, oo[u], oo[u + 1], [&](int w) {
                 if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&cic[w], 1u);
-            }
-            for (unsigned long long e = io[u]; e < io[u + 1]; ++e) {
-                int w = P->in[e];
+            });
+            thread_scan(P->in, io[u], io[u + 1], [&](int w) {
                 if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&coc[w], 1u);
-            }
+            });
             bytes += 4ull * ((oo[u + 1] - oo[u]) + (io[u + 1] - io[u])) + 32;
         }
         __syncthreads();
     }
     // count valid candidates (old bounds) into the member-degree histograms
-    __device__ int hist_valid(const int32_t* R, int nr, int cm, int ub, int lb, int meo, int mei) {
+    __device__ __forceinline__ int hist_valid(const int32_t* R, int nr, int cm, int ub, int lb, int meo, int mei) {
         clear_hist(cm);
         __syncthreads();
         int c = 0;
@@ -442,7 +494,7 @@ struct W {
     // histograms); ccount is the child's candidate count before that pruning. Degrees are
     // recounted exactly over the remaining set (the reference's recount and decrement
     // branches produce the same values). On success *R / *nr hold the final candidates.
-    __device__ bool degree_prune(int cm, int32_t** R, int* nr, int ccount, int& ub, int& lb,
+    __device__ __forceinline__ bool degree_prune(int cm, int32_t** R, int* nr, int ccount, int& ub, int& lb,
                                  int& meo, int& mei, int32_t* spare, int32_t* dropped) {
         int rc = *nr;
         if (!lu_bounds(cm, rc, ub, lb, meo, mei)) return false;
@@ -471,7 +523,7 @@ struct W {
     }
 
     // reference src/device_kernels.cu:1488-1532 (d_check_for_clique)
-    __device__ void check_clique(int cm) {
+    __device__ __forceinline__ void check_clique(int cm) {
         if (cm < P->min_size) return;
         uint32_t mo = (uint32_t)P->mdo[cm], mi = (uint32_t)P->mdi[cm];
         int bad = 0;
@@ -483,7 +535,7 @@ struct W {
         emit_clique(mem, cm);
     }
 
-    __device__ void emit_clique(const int32_t* ids, int len) {
+    __device__ __forceinline__ void emit_clique(const int32_t* ids, int len) {
         unsigned long long a = 0;
         if (tid == 0) a = atomicAdd(&P->g->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
         a = bcast0(a);
@@ -504,7 +556,7 @@ struct W {
     // Orders members then candidates by the enumeration order of reference
     // inc/device_kernels.hpp:31-80 (min(mem) desc, min(can) desc, lvl2adj desc, id desc)
     // and pushes the child onto the task stack.
-    __device__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
+    __device__ __forceinline__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
         const int N = cm + nr;
         // key field widths from this child's maxima
         uint32_t mmx = 0, mcx = 0, l2x = 0;
@@ -540,9 +592,10 @@ struct W {
         const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
         TaskDesc* dd = P->sp_desc;
         dqc::TaskVertex* dv = P->sp_vtx + o;
-        if (total <= 64 && N <= SORT_N) {
+        if (total <= 64) {
             int Np = 1;
             while (Np < N) Np <<= 1;
+            unsigned long long* sk = Np <= SORT_N ? S->sortk : hsort;
             for (int i = tid; i < Np; i += NT) {
                 unsigned long long key = ~0ull;
                 if (i < N) {
@@ -558,7 +611,7 @@ struct W {
                     key = (key << bl) | l2;
                     key = (key << bi) | id;
                 }
-                S->sortk[i] = key;
+                sk[i] = key;
             }
             __syncthreads();
             for (int k = 2; k <= Np; k <<= 1)
@@ -566,20 +619,20 @@ struct W {
                     for (int i = tid; i < Np; i += NT) {
                         int ixj = i ^ j;
                         if (ixj > i) {
-                            unsigned long long x = S->sortk[i], y = S->sortk[ixj];
+                            unsigned long long x = sk[i], y = sk[ixj];
                             bool up = (i & k) == 0;
-                            if ((x > y) == up) { S->sortk[i] = y; S->sortk[ixj] = x; }
+                            if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
                         }
                     }
                     __syncthreads();
                 }
             const unsigned long long idmask = bi == 0 ? 0ull : ((1ull << bi) - 1);
             for (int i = tid; i < N; i += NT) {
-                int x = (int)(idx_max - (uint32_t)(S->sortk[i] & idmask));
+                int x = (int)(idx_max - (uint32_t)(sk[i] & idmask));
                 dv[i] = dqc::TaskVertex{(uint32_t)x, com[x], coc[x], cim[x], cic[x], pl2[x]};
             }
         } else {
-            // fallback: rank by full comparison (rare: very large children)
+            // fallback: rank by full comparison (keys wider than 64 bits)
             if (tid == 0) atomicAdd(&P->g->slow_sorts, 1ull);
             for (int i = tid; i < N; i += NT) {
                 int x = gat[i];
@@ -614,7 +667,7 @@ struct W {
     // ---------------------------------------------------------------- branch steps
     // reference src/device_kernels.cu:400-531 / src/host_functions.cu:1386-1500.
     // Returns 0 (stop the task), 2 (skip this branch) or 1; a member falling short wins.
-    __device__ int remove_one(int& tot, int& nc, int nm) {
+    __device__ __forceinline__ int remove_one(int& tot, int& nc, int nm) {
         --nc;
         --tot;
         if (tot < P->min_size) return 0;
@@ -622,40 +675,37 @@ struct W {
         const int p = pids[tot], last = pids[tot - 1];
         if (tid == 0) bcl(S->plive, p);
         __syncthreads();
+        // The out-, in- and two-hop lists touch disjoint fields (in_can, out_can, lvl2adj),
+        // so they are walked in one phase. The reference stops after the out-list when a
+        // member falls short; here the in- and two-hop decrements may also have run, which
+        // is unobservable because a failure ends the task.
         int fail = 0, skip = 0;
-        for (unsigned long long e = P->oo[p] + tid; e < P->oo[p + 1]; e += NT) {
-            int w = P->on[e];
+        wg_scan(P->on, P->oo[p], P->oo[p + 1], [&](int w) {
             if (bt(S->plive, w)) {
                 uint32_t ic = pic[w] - 1u;
                 pic[w] = ic;
                 if (pim[w] + ic < mi) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
             }
-        }
-        if (tid == 0) bytes += 4ull * (P->oo[p + 1] - P->oo[p]) + 16;
-        if (__syncthreads_or(fail)) return 0;
-        for (unsigned long long e = P->io[p] + tid; e < P->io[p + 1]; e += NT) {
-            int w = P->in[e];
+        });
+        wg_scan(P->in, P->io[p], P->io[p + 1], [&](int w) {
             if (bt(S->plive, w)) {
                 uint32_t oc = poc[w] - 1u;
                 poc[w] = oc;
                 if (pom[w] + oc < mo) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
             }
-        }
-        if (tid == 0) bytes += 4ull * (P->io[p + 1] - P->io[p]) + 16;
-        if (__syncthreads_or(fail)) return 0;
-        const int sk = __syncthreads_or(skip);
-        for (unsigned long long e = P->to[p] + tid; e < P->to[p + 1]; e += NT) {
-            int w = P->tn[e];
+        });
+        wg_scan(P->tn, P->to[p], P->to[p + 1], [&](int w) {
             if (bt(S->plive, w) && !bt(S->pmem, w)) pl2[w] -= 1u;
-        }
-        if (tid == 0) bytes += 4ull * (P->to[p + 1] - P->to[p]) + 16;
-        __syncthreads();
-        return sk ? 2 : 1;
+        });
+        if (tid == 0)
+            bytes += 4ull * ((P->oo[p + 1] - P->oo[p]) + (P->io[p + 1] - P->io[p]) + (P->to[p + 1] - P->to[p])) + 48;
+        if (__syncthreads_or(fail)) return 0;
+        return __syncthreads_or(skip) ? 2 : 1;
     }
 
     // reference src/host_functions.cu:1333-1383 (host: lvl2adj >= tot - 1) and
     // src/device_kernels.cu:330-397 (device: lvl2adj >= num_cand - 1)
-    __device__ bool lookahead(int tot, int nm, int nc, int mode) {
+    __device__ __forceinline__ bool lookahead(int tot, int nm, int nc, int mode) {
         const uint32_t mo = (uint32_t)gmd(tot, P->mdo), mi = (uint32_t)gmd(tot, P->mdi);
         const uint32_t l2min = mode == MODE_DEVICE ? (uint32_t)(nc - 1) : (uint32_t)(tot - 1);
         for (int base = 0; base < tot; base += NT) {
@@ -674,7 +724,7 @@ struct W {
     // check, child write. Only members and the candidates in L(v) are materialised: all
     // others are dropped by diameter pruning before any of their values is read.
     // Returns true if a child was written.
-    __device__ bool branch(int tot, int nm, int mode, bool first_child_host) {
+    __device__ __forceinline__ bool branch(int tot, int nm, int mode, bool first_child_host) {
         const int v = pids[tot - 1];
         const int cm = nm + 1;
         if (cm + 1 >= HMAX) {  // member histograms sized for fewer members
@@ -690,32 +740,29 @@ struct W {
             bs(S->cmem, x);
         }
         // candidates within two hops of v (the others fail diameter pruning)
-        for (unsigned long long e = P->to[v] + tid; e < P->to[v + 1]; e += NT) {
-            int w = P->tn[e];
+        wg_scan(P->tn, P->to[v], P->to[v + 1], [&](int w) {
             if (bt(S->plive, w) && !bt(S->pmem, w) && w != v) {
                 com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
                 bs(S->cL, w);
                 lst[atomicAdd(&S->cnt[3], 1)] = w;
             }
-        }
+        });
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
         __syncthreads();
         const int nl = S->cnt[3];
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
-        for (unsigned long long e = P->oo[v] + tid; e < P->oo[v + 1]; e += NT) {
-            int w = P->on[e];
+        wg_scan(P->on, P->oo[v], P->oo[v + 1], [&](int w) {
             if (bt(S->cmem, w) || bt(S->cL, w)) { cim[w] += 1u; cic[w] -= 1u; }
-        }
-        for (unsigned long long e = P->io[v] + tid; e < P->io[v + 1]; e += NT) {
-            int w = P->in[e];
+        });
+        wg_scan(P->in, P->io[v], P->io[v + 1], [&](int w) {
             if (bt(S->cmem, w) || bt(S->cL, w)) { com[w] += 1u; coc[w] -= 1u; }
-        }
+        });
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
         return branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1) >= 0;
     }
 
-    __device__ void begin_child(int cm) {
+    __device__ __forceinline__ void begin_child(int cm) {
         for (int w = tid; w < kw; w += NT) { S->cmem[w] = 0; S->ccand[w] = 0; S->cL[w] = 0; S->ccrit[w] = 0; }
         if (tid == 0) { S->cnt[3] = 0; S->cnt[4] = 0; }
         clear_hist(cm);
@@ -728,7 +775,7 @@ struct W {
     // needs is a count on a sorted list: can-degrees count neighbours below v,
     // lvl2adj loses one per removed vertex (id >= k - s) whose two-hop list holds the
     // vertex (reference src/host_functions.cu:1386-1500, 1503-1559).
-    __device__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
+    __device__ __forceinline__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
         const int v = k - s - 1, tot = k - s;
         begin_child(1);
         const unsigned long long *oo = P->oo, *io = P->io;
@@ -762,7 +809,7 @@ struct W {
     // Diameter test through child write (reference src/device_kernels.cu:122-165 /
     // src/host_functions.cu:1067-1133). Members are mem[0..nm] (v last), the L(v)
     // candidates lst[0..nl) with post-add degrees. Returns the child's stack index or -1.
-    __device__ long long branch_tail(int nm, int tot, int nl, int mode, int child_mode, int split_index) {
+    __device__ __forceinline__ long long branch_tail(int nm, int tot, int nl, int mode, int child_mode, int split_index) {
         int cm = nm + 1;
         // reference src/device_kernels.cu:830-906 (d_diameter_pruning): basic degree test
         {
@@ -807,22 +854,20 @@ struct W {
     // diameter pruning by the critical set). Returns 2 (critical failure), 0 or 1.
     // cm / R / nr are updated in place.
     // On entry R is la or lb (degree_prune alternates those two and drops into lc).
-    __device__ int critical(int& cm, int32_t*& R, int& nr, int& ub, int& lbd, int& meo, int& mei) {
+    __device__ __forceinline__ int critical(int& cm, int32_t*& R, int& nr, int& ub, int& lbd, int& meo, int& mei) {
         const uint32_t tho = (uint32_t)P->mdo[cm + lbd], thi = (uint32_t)P->mdi[cm + lbd];
         for (int a = 0; a < cm; ++a) {
             int x = mem[a];
             if (com[x] + coc[x] == tho && coc[x] > 0u) {
-                for (unsigned long long e = P->oo[x] + tid; e < P->oo[x + 1]; e += NT) {
-                    int w = P->on[e];
+                wg_scan(P->on, P->oo[x], P->oo[x + 1], [&](int w) {
                     if (bt(S->ccand, w)) bs(S->ccrit, w);
-                }
+                });
                 if (tid == 0) bytes += 4ull * (P->oo[x + 1] - P->oo[x]) + 16;
             }
             if (cim[x] + cic[x] == thi && cic[x] > 0u) {
-                for (unsigned long long e = P->io[x] + tid; e < P->io[x + 1]; e += NT) {
-                    int w = P->in[e];
+                wg_scan(P->in, P->io[x], P->io[x + 1], [&](int w) {
                     if (bt(S->ccand, w)) bs(S->ccrit, w);
-                }
+                });
                 if (tid == 0) bytes += 4ull * (P->io[x + 1] - P->io[x]) + 16;
             }
         }
@@ -850,10 +895,9 @@ struct W {
         // spread over the workgroup (entries of one list are distinct, so plain adds)
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
-            for (unsigned long long e = P->to[c] + tid; e < P->to[c + 1]; e += NT) {
-                const int w = P->tn[e];
+            wg_scan(P->tn, P->to[c], P->to[c + 1], [&](int w) {
                 if (bt(S->cmem, w) || bt(S->ccand, w)) adjc[w] += 1u;
-            }
+            });
             if (tid == 0) bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
             __syncthreads();
         }
@@ -864,14 +908,12 @@ struct W {
         // the critical vertices join the members (reference src/device_kernels.cu:772-805)
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
-            for (unsigned long long e = P->oo[c] + tid; e < P->oo[c + 1]; e += NT) {
-                const int w = P->on[e];
+            wg_scan(P->on, P->oo[c], P->oo[c + 1], [&](int w) {
                 if (bt(S->cmem, w) || bt(S->ccand, w)) { cim[w] += 1u; cic[w] -= 1u; }
-            }
-            for (unsigned long long e = P->io[c] + tid; e < P->io[c + 1]; e += NT) {
-                const int w = P->in[e];
+            });
+            wg_scan(P->in, P->io[c], P->io[c + 1], [&](int w) {
                 if (bt(S->cmem, w) || bt(S->ccand, w)) { com[w] += 1u; coc[w] -= 1u; }
-            }
+            });
             if (tid == 0) bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
             __syncthreads();
         }
@@ -909,7 +951,7 @@ struct W {
     // ---------------------------------------------------------------- one task
     // reference src/device_kernels.cu:25-173 (device levels) and
     // src/host_functions.cu:987-1134 (host levels 0 and 1)
-    __device__ void run_task(const TaskDesc& d, const dqc::TaskVertex* rv) {
+    __device__ __forceinline__ void run_task(const TaskDesc& d, const dqc::TaskVertex* rv) {
         const int T = d.tot, nm = d.nm, mode = d.mode;
         const int ncov = mode == MODE_DEVICE ? 0 : d.ncov;
         for (int i = tid; i < T; i += NT) {
@@ -950,7 +992,12 @@ struct W {
     }
 };
 
-__global__ __launch_bounds__(NT) void expand_kernel(KParams P) {
+// Register budget: DQC_WAVES_PER_EU asks the compiler for that many waves per SIMD.
+#ifndef DQC_WAVES_PER_EU
+#define DQC_WAVES_PER_EU 8
+#endif
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DQC_WAVES_PER_EU)))
+void expand_kernel(KParams P) {
     __shared__ Shared S;
     W w;
     w.P = &P;
@@ -967,6 +1014,7 @@ __global__ __launch_bounds__(NT) void expand_kernel(KParams P) {
     int32_t* lists = (int32_t*)(base + 10 * k);
     w.pids = lists; w.mem = lists + k; w.la = lists + 2 * k; w.lb = lists + 3 * k; w.lc = lists + 4 * k;
     w.lst = lists + 5 * k; w.crit = lists + 6 * k; w.gat = lists + 7 * k;
+    w.hsort = (unsigned long long*)(lists + 8 * k);
     for (int i = threadIdx.x; i < w.kw; i += NT) { S.plive[i] = 0; S.pmem[i] = 0; }
     if (threadIdx.x == 0) S.acc_bytes = 0;
     __syncthreads();
@@ -1237,8 +1285,9 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
             (m_two && !hip_ok(h, hipMemcpy(h->d_ltn, ltn.data(), sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload")))
             return DQC_E_HIP;
     }
-    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each
-    h->stride = 18ull * (unsigned long long)std::max(n, 1);
+    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each, and
+    // the HBM sort area (2k 64-bit keys: a child has at most k vertices, padded to a power of 2)
+    h->stride = 22ull * (unsigned long long)std::max(n, 1);
     if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(h->workgroups + 1), "scratch alloc")) return DQC_E_HIP;
     h->st.n_condensed = n;
     return DQC_OK;
