@@ -30,7 +30,12 @@ class Config(ctypes.Structure):
         ("spill_tasks", ctypes.c_uint64), ("spill_vertices", ctypes.c_uint64),
         ("clique_sets", ctypes.c_uint64), ("clique_ids", ctypes.c_uint64),
         ("host_addfail_check", ctypes.c_int32),
+        ("split_branches", ctypes.c_int32), ("split_chunk", ctypes.c_int32),
+        ("schedule", ctypes.c_int32),
     ]
+
+
+SCHED_DEFAULT, SCHED_FIFO, SCHED_SIZE, SCHED_LEARNED = 0, 1, 2, 3
 
 
 class Stats(ctypes.Structure):
@@ -145,13 +150,13 @@ class Engine:
                                               ctypes.byref(k)), "dqc_prepare_file")
         return k.value
 
-    def step(self, launches=1):
+    def step(self, launches=1, sched=SCHED_DEFAULT):
         fin = ctypes.c_int32()
-        self._check(self.lib.dqc_step(self.h, 0, int(launches), ctypes.byref(fin)), "dqc_step")
+        self._check(self.lib.dqc_step(self.h, int(sched), int(launches), ctypes.byref(fin)), "dqc_step")
         return bool(fin.value)
 
-    def run(self):
-        self._check(self.lib.dqc_run(self.h, 0), "dqc_run")
+    def run(self, sched=SCHED_DEFAULT):
+        self._check(self.lib.dqc_run(self.h, int(sched)), "dqc_run")
 
     def stats(self):
         st = Stats()

@@ -52,15 +52,28 @@ constexpr int HMAX = 256;          // member-degree histogram bins (members < HM
 #define DQC_SCAN_UNR 8
 #endif
 constexpr int SCAN_UNR = DQC_SCAN_UNR;  // list entries in flight per thread in list scans
-constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2, MODE_SKIP = 3;
-constexpr unsigned long long CNT_SHIFT = 40;  // combined allocation: (count << 40) | size
+constexpr int MODE_HOST_L0 = 0, MODE_HOST_L1 = 1, MODE_DEVICE = 2;
+// Combined allocation counters: (count << 34) | size. Capacities are held below 2^30
+// entries and 2^33 records / ids, so neither field can carry into the other even when
+// a full buffer keeps receiving (failing) requests from every workgroup.
+constexpr unsigned long long CNT_SHIFT = 34;
 constexpr unsigned long long SIZE_MASK = (1ull << CNT_SHIFT) - 1;
+constexpr unsigned long long MAX_ENTRIES = (1ull << 30) - 1;
+constexpr unsigned long long MAX_RECORDS = 1ull << 33;
 
 enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u };
+
+// flags of a task descriptor
+constexpr int32_t TF_PIECE = 1;         // a piece of a split task: not counted, not split again
+constexpr int32_t TF_REMOVE_FIRST = 2;  // the piece's first step starts with a removal
+constexpr int32_t TF_SPLIT = 4;         // set by the batch scheduler: cut this task into pieces
+constexpr int MAX_PIECES = 64;          // pieces per split task (the chunk grows beyond that)
 
 struct TaskDesc {
     unsigned long long off;
     int32_t tot, nm, ncov, mode;
+    int32_t iters;  // loop steps to run (pieces); whole tasks run tot - nm - ncov
+    int32_t flags;
 };
 
 // Per-queue counters (one per task stack / stream).
@@ -102,6 +115,7 @@ struct KParams {
     const int32_t* ltn;
     uint32_t* scratch;
     unsigned long long stride;  // uint32 words per workgroup
+    int32_t split_c;            // least branches per piece of a split task
 };
 
 struct Shared {
@@ -537,8 +551,11 @@ struct W {
 
     __device__ __forceinline__ void emit_clique(const int32_t* ids, int len) {
         unsigned long long a = 0;
-        if (tid == 0) a = atomicAdd(&P->g->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
+        if (tid == 0)
+            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull
+                : atomicAdd(&P->g->clq_alloc, (1ull << CNT_SHIFT) | (unsigned long long)len);
         a = bcast0(a);
+        if (a == ~0ull) return;
         unsigned long long idx = a >> CNT_SHIFT, off = a & SIZE_MASK;
         if (idx >= P->clq_cap_s || off + len > P->clq_cap_i) {
             if (tid == 0) atomicOr(&P->g->error, (unsigned)ERR_CLIQUES);
@@ -581,8 +598,8 @@ struct W {
         unsigned long long a = 0;
         if (tid == 0) {
             const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)N;
-            a = atomicAdd(&P->q->sp_alloc, inc);
-            if ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + N > P->sp_cap_v) {
+            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(&P->q->sp_alloc, inc);
+            if (a != ~0ull && ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + N > P->sp_cap_v)) {
                 atomicOr(&P->g->error, (unsigned)ERR_SPILL);
                 a = ~0ull;
             }
@@ -657,11 +674,43 @@ struct W {
         if (tid == 0) {
             TaskDesc d;
             d.off = o; d.tot = N; d.nm = cm; d.ncov = 0; d.mode = mode;
+            d.iters = N - cm; d.flags = 0;
             dd[t] = d;
         }
         bytes += 24ull * N / NT + 1;
         __syncthreads();
         return (long long)t;
+    }
+
+    // Pushes the current state of the task (live vertices in position order with their
+    // current degrees and lvl2adj) as a piece that runs `iters` loop steps from here.
+    __device__ __forceinline__ void write_piece(int tot, int nm, int mode, bool remove_first, int iters) {
+        unsigned long long a = 0;
+        if (tid == 0) {
+            const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)tot;
+            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(&P->q->sp_alloc, inc);
+            if (a != ~0ull && ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + tot > P->sp_cap_v)) {
+                atomicOr(&P->g->error, (unsigned)ERR_SPILL);
+                a = ~0ull;
+            }
+        }
+        a = bcast0(a);
+        if (a == ~0ull) return;
+        const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
+        dqc::TaskVertex* dv = P->sp_vtx + o;
+        for (int i = tid; i < tot; i += NT) {
+            const int x = pids[i];
+            dv[i] = dqc::TaskVertex{(uint32_t)x, pom[x], poc[x], pim[x], pic[x], pl2[x]};
+        }
+        if (tid == 0) {
+            TaskDesc d;
+            d.off = o; d.tot = tot; d.nm = nm; d.ncov = 0; d.mode = mode;
+            d.iters = iters;
+            d.flags = TF_PIECE | (remove_first ? TF_REMOVE_FIRST : 0);
+            P->sp_desc[t] = d;
+        }
+        bytes += 24ull * tot / NT + 1;
+        __syncthreads();
     }
 
     // ---------------------------------------------------------------- branch steps
@@ -951,9 +1000,16 @@ struct W {
     // ---------------------------------------------------------------- one task
     // reference src/device_kernels.cu:25-173 (device levels) and
     // src/host_functions.cu:987-1134 (host levels 0 and 1)
+    //
+    // A task with many branches is split instead: one pass runs only the removal and
+    // lookahead steps, and every split_c steps pushes the state as a piece that runs the
+    // next split_c steps with full branches (its first step removes unless it is the
+    // task's first). The pieces run exactly the branches of the sequential loop, so the
+    // branches, cliques and children are the same; the task is counted once, here.
     __device__ __forceinline__ void run_task(const TaskDesc& d, const dqc::TaskVertex* rv) {
         const int T = d.tot, nm = d.nm, mode = d.mode;
         const int ncov = mode == MODE_DEVICE ? 0 : d.ncov;
+        const bool piece = (d.flags & TF_PIECE) != 0;
         for (int i = tid; i < T; i += NT) {
             dqc::TaskVertex r = rv[i];
             int x = (int)r.id;
@@ -963,13 +1019,20 @@ struct W {
             if (i < nm) bs(S->pmem, x);
         }
         bytes += 24ull * T / NT + 1;
-        if (tid == 0) atomicAdd(&P->g->tasks, 1ull);
+        if (tid == 0 && !piece) atomicAdd(&P->g->tasks, 1ull);
         __syncthreads();
         int tot = T, nc = T - nm;
-        const int expansions = nc;
+        const int expansions = piece ? min(d.iters, nc) : nc;
+        // the batch scheduler marks the tasks to split (host side, within a record budget)
+        const int steps = expansions - ncov;
+        const int chunk = max(P->split_c, (steps + MAX_PIECES - 1) / MAX_PIECES);
+        const bool split = !piece && mode != MODE_HOST_L0 && (d.flags & TF_SPLIT);
+        const bool remove_first = piece && (d.flags & TF_REMOVE_FIRST);
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
-            if (j != ncov) {
+            if (split && (j - ncov) % chunk == 0)
+                write_piece(tot, nm, mode, j != ncov, min(chunk, expansions - j));
+            if (j != ncov || remove_first) {
                 tick(0);
                 int r = remove_one(tot, nc, nm);
                 tick(1);
@@ -980,9 +1043,10 @@ struct W {
             bool la = lookahead(tot, nm, nc, mode);
             tick(2);
             if (la) {
-                emit_clique(pids, tot);
+                if (!split) emit_clique(pids, tot);  // the piece holding this step emits it
                 break;
             }
+            if (split) continue;
             bool wrote = branch(tot, nm, mode, first_host_child);
             tick(0);
             if (wrote) first_host_child = false;
@@ -1028,7 +1092,7 @@ void expand_kernel(KParams P) {
             w.branch_root(P.split_s[t], d.tot, P.in_vtx + d.off);
         } else {
             TaskDesc d = P.in_desc[t];
-            if (d.mode != MODE_SKIP) w.run_task(d, P.in_vtx + d.off);  // lifted to the side stream
+            w.run_task(d, P.in_vtx + d.off);
         }
     }
     atomicAdd(&S.acc_bytes, w.bytes);
@@ -1042,7 +1106,8 @@ void expand_kernel(KParams P) {
     }
 }
 
-// Copies tasks [first, first + m) of the task stack into the batch buffer.
+// Copies the records of tasks [first, first + m) of the task stack into the batch buffer
+// (and their descriptors, rebased, when dst_desc is given).
 __global__ void refill_kernel(const TaskDesc* sp_desc, const dqc::TaskVertex* sp_vtx,
                               unsigned long long first, unsigned long long m,
                               TaskDesc* dst_desc, dqc::TaskVertex* dst_vtx,
@@ -1051,11 +1116,12 @@ __global__ void refill_kernel(const TaskDesc* sp_desc, const dqc::TaskVertex* sp
     const unsigned long long vend = sp_desc[first + m - 1].off + (unsigned long long)sp_desc[first + m - 1].tot;
     const unsigned long long gid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned long long gsz = (unsigned long long)gridDim.x * blockDim.x;
-    for (unsigned long long i = gid; i < m; i += gsz) {
-        TaskDesc d = sp_desc[first + i];
-        d.off = d.off - v0 + dst_v0;
-        dst_desc[dst_t0 + i] = d;
-    }
+    if (dst_desc)
+        for (unsigned long long i = gid; i < m; i += gsz) {
+            TaskDesc d = sp_desc[first + i];
+            d.off = d.off - v0 + dst_v0;
+            dst_desc[dst_t0 + i] = d;
+        }
     for (unsigned long long i = gid; i < vend - v0; i += gsz) dst_vtx[dst_v0 + i] = sp_vtx[v0 + i];
 }
 
@@ -1127,14 +1193,6 @@ struct dqc_handle {
     unsigned long long stride = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // side stream: the host level-1 task runs there while the main batches continue
-    hipStream_t side = nullptr;
-    QCtr* d_q2 = nullptr;
-    TaskDesc* d_s2desc = nullptr;       // side task stack (children of the lifted task)
-    dqc::TaskVertex* d_s2vtx = nullptr;
-    TaskDesc* d_sbdesc = nullptr;       // side batch buffer (the lifted task)
-    dqc::TaskVertex* d_sbvtx = nullptr;
-    bool side_running = false;
     dqc_stats st{};
     unsigned long long max_depth = 0;
     unsigned long long last_root_first = ~0ull;
@@ -1142,6 +1200,8 @@ struct dqc_handle {
     bool root_pending = false;
     int profile = 0;
     unsigned long long phase[8] = {0};
+    std::vector<TaskDesc> hbatch;  // scheduled batch descriptors (host copy for the upload)
+    std::vector<double> hcost;
     std::vector<void*> allocs;
 };
 
@@ -1191,7 +1251,17 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!c.spill_vertices) c.spill_vertices = 1ull << 30;
     if (!c.clique_sets) c.clique_sets = 1ull << 25;
     if (!c.clique_ids) c.clique_ids = 1ull << 29;
+    if (!c.split_branches) c.split_branches = 1024;
+    if (c.split_chunk <= 0) c.split_chunk = 32;
+    if (c.schedule == DQC_SCHED_DEFAULT) c.schedule = DQC_SCHED_SIZE;
     h->device = c.device;
+    *out = h;
+    if (c.spill_tasks > MAX_ENTRIES || c.clique_sets > MAX_ENTRIES || c.spill_vertices > MAX_RECORDS ||
+        c.clique_ids > MAX_RECORDS || c.pool_vertices > MAX_RECORDS || c.schedule < DQC_SCHED_FIFO ||
+        c.schedule > DQC_SCHED_LEARNED) {
+        h->err = "capacity above 2^30 tasks / cliques or 2^33 records / ids, or unknown schedule";
+        return DQC_E_ARG;
+    }
     h->profile = std::getenv("DQC_PHASES") != nullptr;
     *out = h;
     if (!hip_ok(h, hipSetDevice(h->device), "hipSetDevice")) return DQC_E_HIP;
@@ -1204,9 +1274,8 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (per_cu < 1) per_cu = 1;
     h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
     if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
-    // non-blocking: the main loop's synchronous copies must not wait for the side stream
+    // non-blocking stream: every host read of its results synchronizes on it explicitly
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
-    if (!hip_ok(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev0), "hipEventCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev1), "hipEventCreate")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_bdesc, c.pool_tasks, "batch alloc")) return DQC_E_HIP;
@@ -1215,12 +1284,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!dev_alloc(h, &h->d_svtx, c.spill_vertices, "stack alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_clq_ids, c.clique_ids, "clique alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_clq_meta, c.clique_sets, "clique alloc")) return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_q, 1, "counter alloc") || !dev_alloc(h, &h->d_g, 1, "counter alloc") ||
-        !dev_alloc(h, &h->d_q2, 1, "counter alloc"))
-        return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_s2desc, c.spill_tasks / 4 + 1, "side alloc") ||
-        !dev_alloc(h, &h->d_s2vtx, c.pool_vertices, "side alloc") || !dev_alloc(h, &h->d_sbdesc, 1, "side alloc") ||
-        !dev_alloc(h, &h->d_sbvtx, c.pool_vertices, "side alloc"))
+    if (!dev_alloc(h, &h->d_q, 1, "counter alloc") || !dev_alloc(h, &h->d_g, 1, "counter alloc"))
         return DQC_E_HIP;
     if (!hip_ok(h, hipMemset(h->d_q, 0, sizeof(QCtr)), "hipMemset") ||
         !hip_ok(h, hipMemset(h->d_g, 0, sizeof(GCtr)), "hipMemset"))
@@ -1232,13 +1296,11 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
 dqc_status dqc_close(dqc_handle* h) {
     if (!h) return DQC_E_ARG;
     hipSetDevice(h->device);
-    if (h->side) hipStreamSynchronize(h->side);
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) hipFree(p);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
-    if (h->side) hipStreamDestroy(h->side);
     delete h;
     return DQC_OK;
 }
@@ -1329,6 +1391,7 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
     TaskDesc d;
     d.off = h->st_v;
     d.tot = nvert; d.nm = 0; d.ncov = ncov; d.mode = MODE_HOST_L0;
+    d.iters = nvert - ncov; d.flags = 0;
     if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
         !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
         return DQC_E_HIP;
@@ -1354,6 +1417,7 @@ dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, 
     TaskDesc d;
     d.off = h->st_v;
     d.tot = nvert; d.nm = nm; d.ncov = ncov; d.mode = mode;
+    d.iters = nvert - nm - ncov; d.flags = 0;
     if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
         !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
         return DQC_E_HIP;
@@ -1384,8 +1448,6 @@ dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, in
 dqc_status dqc_reset(dqc_handle* h) {
     if (!h) return DQC_E_ARG;
     hipSetDevice(h->device);
-    if (h->side) hipStreamSynchronize(h->side);
-    h->side_running = false;
     h->st_t = 0;
     h->st_v = 0;
     h->root_pending = false;
@@ -1398,9 +1460,17 @@ dqc_status dqc_reset(dqc_handle* h) {
     return DQC_OK;
 }
 
+static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, int32_t sched, double* cost);
+
 // Pops the newest tasks off the stack into the batch buffer (as many as the batch
 // holds); returns the number popped (0 with an error set when none fits).
-static dqc_status pop_batch(dqc_handle* h, unsigned long long* popped) {
+//
+// FIFO keeps stack order. SIZE and LEARNED order the batch by an estimate of each
+// task's cost, largest first, so that the longest tasks start first, and mark the
+// costliest tasks with at least split_branches branches for splitting, within a
+// budget of pool_vertices / 16 piece records per batch (pieces copy the task's records
+// and run all of its branches at once, so the budget bounds the stack growth).
+static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* popped) {
     unsigned long long m = std::min<unsigned long long>(h->st_t, h->cfg.pool_tasks);
     std::vector<TaskDesc> tail(m);
     if (!hip_ok(h, hipMemcpy(tail.data(), h->d_sdesc + (h->st_t - m), sizeof(TaskDesc) * m, hipMemcpyDeviceToHost), "stack read"))
@@ -1410,11 +1480,57 @@ static dqc_status pop_batch(dqc_handle* h, unsigned long long* popped) {
     while (skip < m && vend - tail[skip].off > h->cfg.pool_vertices) ++skip;
     if (skip == m) { h->err = "a task does not fit the batch buffer (raise pool_vertices)"; return DQC_E_CAPACITY; }
     const unsigned long long first = h->st_t - m + skip, cnt = m - skip, v0 = tail[skip].off;
-    refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
+    if (sched == DQC_SCHED_FIFO) {
+        refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
+    } else {
+        const TaskDesc* b = tail.data() + skip;
+        h->hcost.resize(cnt);
+        dqc_status rc = batch_costs(h, b, cnt, sched, h->hcost.data());
+        if (rc != DQC_OK) return rc;
+        std::vector<uint32_t> order(cnt);
+        for (uint32_t i = 0; i < cnt; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h->hcost[x] > h->hcost[y]; });
+        h->hbatch.resize(cnt);
+        unsigned long long budget = h->cfg.pool_vertices / 16;
+        if (h->st_v + budget > h->cfg.spill_vertices) budget = h->cfg.spill_vertices > h->st_v ? (h->cfg.spill_vertices - h->st_v) / 2 : 0;
+        for (unsigned long long i = 0; i < cnt; ++i) {
+            TaskDesc d = b[order[i]];
+            d.off -= v0;
+            const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
+            if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && h->cfg.split_branches > 0 &&
+                steps >= h->cfg.split_branches) {
+                const int chunk = std::max(h->cfg.split_chunk, (steps + MAX_PIECES - 1) / MAX_PIECES);
+                const unsigned long long need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
+                if (need <= budget) {
+                    budget -= need;
+                    d.flags |= TF_SPLIT;
+                }
+            }
+            h->hbatch[i] = d;
+        }
+        if (!hip_ok(h, hipMemcpyAsync(h->d_bdesc, h->hbatch.data(), sizeof(TaskDesc) * cnt, hipMemcpyHostToDevice, h->stream),
+                    "batch upload"))
+            return DQC_E_HIP;
+        refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, nullptr, h->d_bvtx, 0, 0);
+    }
     if (!hip_ok(h, hipGetLastError(), "pop launch")) return DQC_E_HIP;
     h->st_t = first;
     h->st_v = v0;
     *popped = cnt;
+    return DQC_OK;
+}
+
+// Cost estimates of a batch: SIZE uses branches x task size (every branch scans lists
+// of vertices of the task); LEARNED uses the handle's cost model.
+static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, int32_t sched, double* cost) {
+    if (sched == DQC_SCHED_LEARNED) {
+        h->err = "no cost model set (dqc_set_cost_model)";
+        return DQC_E_ARG;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const double steps = (double)(b[i].tot - b[i].nm);
+        cost[i] = ((b[i].flags & TF_PIECE) ? (double)b[i].iters : steps) * (double)b[i].tot;
+    }
     return DQC_OK;
 }
 
@@ -1450,6 +1566,7 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.lto = h->d_lto; P.ltn = h->d_ltn;
     P.scratch = h->d_scratch;
     P.stride = h->stride;
+    P.split_c = h->cfg.split_chunk;
     return P;
 }
 
@@ -1497,71 +1614,6 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     return DQC_OK;
 }
 
-// Lifts task t of the main stack (the host level-1 task) onto the side stream: its
-// descriptor on the main stack is marked skipped, a copy is expanded by one workgroup
-// on the side stream into the side task stack, and merge_side later moves its children
-// onto the main stack. The level-1 branch loop is sequential by construction; this keeps
-// it from holding up the main batches.
-static dqc_status start_side(dqc_handle* h, unsigned long long t) {
-    TaskDesc d;
-    if (!hip_ok(h, hipMemcpy(&d, h->d_sdesc + t, sizeof d, hipMemcpyDeviceToHost), "side read")) return DQC_E_HIP;
-    if ((unsigned long long)d.tot > h->cfg.pool_vertices) return DQC_OK;  // stays on the main stack
-    TaskDesc c = d;
-    c.off = 0;
-    if (!hip_ok(h, hipMemcpy(h->d_sbvtx, h->d_svtx + d.off, sizeof(dqc::TaskVertex) * d.tot, hipMemcpyDeviceToDevice), "side copy") ||
-        !hip_ok(h, hipMemcpy(h->d_sbdesc, &c, sizeof c, hipMemcpyHostToDevice), "side copy"))
-        return DQC_E_HIP;
-    const int32_t skip = MODE_SKIP;
-    if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &skip, sizeof skip, hipMemcpyHostToDevice), "skip mark"))
-        return DQC_E_HIP;
-    QCtr q{};
-    q.root_first = ~0ull;
-    if (!hip_ok(h, hipMemcpy(h->d_q2, &q, sizeof q, hipMemcpyHostToDevice), "side ctr")) return DQC_E_HIP;
-    KParams P = make_params(h, 1);
-    P.in_desc = h->d_sbdesc; P.in_vtx = h->d_sbvtx;
-    P.sp_desc = h->d_s2desc; P.sp_vtx = h->d_s2vtx;
-    P.sp_cap_t = h->cfg.spill_tasks / 4 + 1; P.sp_cap_v = h->cfg.pool_vertices;
-    P.q = h->d_q2;
-    P.scratch = h->d_scratch + h->stride * (unsigned long long)h->workgroups;  // spare slot
-    expand_kernel<<<1, NT, 0, h->side>>>(P);
-    if (!hip_ok(h, hipGetLastError(), "side launch")) return DQC_E_HIP;
-    h->side_running = true;
-    return DQC_OK;
-}
-
-// If the side kernel has finished (or `wait`), moves its children onto the main stack.
-static dqc_status merge_side(dqc_handle* h, bool wait) {
-    if (!h->side_running) return DQC_OK;
-    if (wait) {
-        if (!hip_ok(h, hipStreamSynchronize(h->side), "side kernel")) return DQC_E_HIP;
-    } else {
-        hipError_t e = hipStreamQuery(h->side);
-        if (e == hipErrorNotReady) return DQC_OK;
-        if (!hip_ok(h, e, "side kernel")) return DQC_E_HIP;
-    }
-    h->side_running = false;
-    QCtr q{};
-    GCtr g{};
-    if (!hip_ok(h, hipMemcpy(&q, h->d_q2, sizeof q, hipMemcpyDeviceToHost), "side ctr") ||
-        !hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read"))
-        return DQC_E_HIP;
-    if (g.error) { h->err = "side stream: task stack or clique store full"; return DQC_E_CAPACITY; }
-    fold_stats(h, g);
-    const unsigned long long n2 = q.sp_alloc >> CNT_SHIFT, v2 = q.sp_alloc & SIZE_MASK;
-    if (n2 == 0) return DQC_OK;
-    if (h->st_t + n2 > h->cfg.spill_tasks || h->st_v + v2 > h->cfg.spill_vertices) {
-        h->err = "task stack full (raise spill_tasks / spill_vertices)";
-        return DQC_E_CAPACITY;
-    }
-    refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_s2desc, h->d_s2vtx, 0, n2, h->d_sdesc, h->d_svtx, h->st_t, h->st_v);
-    if (!hip_ok(h, hipGetLastError(), "merge launch") || !hip_ok(h, hipStreamSynchronize(h->stream), "merge"))
-        return DQC_E_HIP;
-    h->st_t += n2;
-    h->st_v += v2;
-    fold_stats(h, g);
-    return DQC_OK;
-}
-
 // The level-0 task expanded branch-parallel across all workgroups (see branch_root):
 // a flags pass finds where the sequential branch loop would stop and which steps skip
 // their branch, then every remaining branch runs as its own work item. The child of the
@@ -1569,7 +1621,7 @@ static dqc_status merge_side(dqc_handle* h, bool wait) {
 // src/host_functions.cu:1110-1133: the first child goes to the CPU level-1 pass).
 static dqc_status split_root(dqc_handle* h) {
     unsigned long long m = 0;
-    dqc_status rc = pop_batch(h, &m);
+    dqc_status rc = pop_batch(h, DQC_SCHED_FIFO, &m);
     if (rc != DQC_OK) return rc;
     // the pop runs on the main stream, which does not order against null-stream copies
     if (!hip_ok(h, hipStreamSynchronize(h->stream), "pop")) return DQC_E_HIP;
@@ -1616,23 +1668,20 @@ static dqc_status split_root(dqc_handle* h) {
         if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &mode, sizeof mode,
                                  hipMemcpyHostToDevice), "mode patch"))
             return DQC_E_HIP;
-        if (std::getenv("DQC_NO_SIDE") == nullptr) return start_side(h, t);
     }
     return DQC_OK;
 }
 
 static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launches, int32_t* finished) {
-    if (!h || sched != 0) return DQC_E_ARG;
+    if (!h || sched < DQC_SCHED_DEFAULT || sched > DQC_SCHED_LEARNED) return DQC_E_ARG;
+    if (sched == DQC_SCHED_DEFAULT) sched = h->cfg.schedule;
     if (!h->d_mdo || !h->d_oo) { h->err = "graph and tables must be set first"; return DQC_E_ARG; }
     hipSetDevice(h->device);
     auto t0 = std::chrono::steady_clock::now();
     const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
     auto last_report = t0;
     int64_t done = 0;
-    while ((h->st_t > 0 || h->side_running) && (max_launches < 0 || done < max_launches)) {
-        dqc_status mrc = merge_side(h, h->st_t == 0);
-        if (mrc != DQC_OK) return mrc;
-        if (h->st_t == 0) continue;
+    while (h->st_t > 0 && (max_launches < 0 || done < max_launches)) {
         ++done;
         if (h->st_t == 1 && h->root_pending) {
             h->root_pending = false;
@@ -1640,7 +1689,7 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
             if (rc != DQC_OK) return rc;
         } else {
             unsigned long long m = 0;
-            dqc_status rc = pop_batch(h, &m);
+            dqc_status rc = pop_batch(h, sched, &m);
             if (rc != DQC_OK) return rc;
             rc = launch(h, make_params(h, m), m);
             if (rc != DQC_OK) return rc;
@@ -1655,14 +1704,14 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
             }
         }
     }
-    if (h->st_t == 0 && !h->side_running) {  // everything done: final counters
+    if (h->st_t == 0) {  // everything done: final counters
         GCtr g{};
         if (!hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
         fold_stats(h, g);
     }
     auto t1 = std::chrono::steady_clock::now();
     h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    *finished = h->st_t == 0 && !h->side_running;
+    *finished = h->st_t == 0;
     return DQC_OK;
 }
 

@@ -30,7 +30,7 @@ typedef struct dqc_handle dqc_handle;
  * (reference src/common.cpp:521-557); 0 selects a default. */
 typedef struct {
     int32_t device;            /* HIP device ordinal                                  */
-    int32_t workgroups;        /* persistent workgroups; 0 = 3 per CU                 */
+    int32_t workgroups;        /* persistent workgroups; 0 = as many as are resident  */
     uint64_t pool_tasks;       /* tasks expanded per launch (batch); 0 = 32 per WG    */
     uint64_t pool_vertices;    /* 24-byte vertex records the batch buffer holds       */
     uint64_t spill_tasks;      /* task descriptors the LIFO task stack holds          */
@@ -39,6 +39,13 @@ typedef struct {
     uint64_t clique_ids;       /* ids of raw cliques held on the device               */
     int32_t host_addfail_check;/* 1 (default 4-rank union) / 0 (single rank, see
                                   reference src/host_functions.cu:1081)                */
+    int32_t split_branches;    /* SIZE / LEARNED schedules: a task with at least this
+                                  many branches may be cut into pieces (at most 64, of
+                                  at least split_chunk branches) that run in parallel;
+                                  0 = 1024, negative = never (output is identical)    */
+    int32_t split_chunk;       /* least branches per piece; 0 = 32                    */
+    int32_t schedule;          /* DQC_SCHED_* used by dqc_search_file and by
+                                  DQC_SCHED_DEFAULT; 0 = DQC_SCHED_SIZE               */
 } dqc_config;
 
 typedef struct {
@@ -80,8 +87,16 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
 
 /* Expands every queued task to completion on the GPU (replaces the level loop of
  * reference src/host_functions.cu:92-285, the d_expand_level / d_transfer_buffers /
- * d_fill_from_buffer kernels and the cuTS MPI stealing). sched: 0 = plain order (batches
- * popped from the top of the LIFO task stack). */
+ * d_fill_from_buffer kernels and the cuTS MPI stealing). sched is one of DQC_SCHED_*;
+ * the enumeration output and the tree counts do not depend on it. */
+#define DQC_SCHED_DEFAULT 0 /* the handle's dqc_config.schedule                      */
+#define DQC_SCHED_FIFO 1    /* plain: batches popped from the top of the LIFO task stack,
+                               expanded in stack order, never split                   */
+#define DQC_SCHED_SIZE 2    /* each batch ordered by a size estimate, largest first; the
+                               costliest tasks (>= split_branches branches) are cut
+                               into pieces                                            */
+#define DQC_SCHED_LEARNED 3 /* as SIZE, with the subtree-cost predictor of
+                               dqc_set_cost_model as the estimate                     */
 dqc_status dqc_run(dqc_handle* h, int32_t sched);
 
 /* Queues any task (same record layout as dqc_push_root): nm members first, then ncov

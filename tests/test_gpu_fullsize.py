@@ -34,9 +34,16 @@ def oracle_tasks(w2):
     t.close()
 
 
-def test_level0_children_match_oracle(dqc, w2, oracle_tasks, monkeypatch):
+def config(dqc, split_branches=-1, split_chunk=0):
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.split_branches = split_branches
+    cfg.split_chunk = split_chunk
+    return cfg
+
+
+def test_level0_children_match_oracle(dqc, w2, oracle_tasks):
     """Branch-parallel level 0 == the reference's sequential level-0 loop, at full size."""
-    monkeypatch.setenv("DQC_NO_SIDE", "1")  # keep the level-1 task on the stack
     _, root, ncov, okids, ocl, obr = oracle_tasks
     with dqc.Engine() as eng:
         k = eng.prepare(w2, GO, GI, MN)
@@ -60,7 +67,7 @@ def test_level1_tasks_match_oracle(dqc, w2, oracle_tasks):
     ot, _, _, okids, _, _ = oracle_tasks
     order = sorted(range(1, len(okids)), key=lambda i: -len(okids[i][0]))
     sample = [0] + order[:3] + list(range(1, len(okids), max(1, len(okids) // 10)))[:10]
-    with dqc.Engine() as eng:
+    with dqc.Engine(config(dqc)) as eng:
         eng.prepare(w2, GO, GI, MN)
         for i in sample:
             recs, nm = okids[i]
@@ -74,5 +81,35 @@ def test_level1_tasks_match_oracle(dqc, w2, oracle_tasks):
             kids, cl, br = ot.expand(recs, nm, 0, mode)
             print(f"task {i}: size {len(recs)}, {len(kids)} children, {br} branches, {len(cl)} cliques", flush=True)
             assert gbr == br, i
+            assert gk == sorted((tuple(r), m) for r, m in kids), i
+            assert gc == cl, i
+
+
+def test_split_tasks_match_oracle(dqc, w2, oracle_tasks):
+    """The host level-1 task and the largest level-1 tasks cut into pieces of 16 branches:
+    one step runs the split pass (pieces only, no branches), the next runs every piece;
+    the children, cliques and branch count must then equal the oracle's unsplit expansion."""
+    ot, _, _, okids, _, _ = oracle_tasks
+    order = sorted(range(1, len(okids)), key=lambda i: -len(okids[i][0]))
+    with dqc.Engine(config(dqc, split_branches=64, split_chunk=16)) as eng:
+        eng.prepare(w2, GO, GI, MN)
+        for i in [0] + order[:3]:
+            recs, nm = okids[i]
+            mode = orc.Tasks.HOST_L1 if i == 0 else orc.Tasks.DEVICE
+            eng.reset()
+            eng.push_task(recs, nm, 0, mode)
+            eng.step(1)
+            pieces = eng.tasks()
+            assert eng.stats().branches == 0 and eng.stats().tasks_expanded == 1
+            # one piece per 16 steps, fewer if the loop stops early (a failed removal or lookahead)
+            assert 1 <= len(pieces) <= -(-(len(recs) - nm) // 16) and all(m == mode for _, _, m in pieces), i
+            eng.step(1)
+            gk = sorted((tuple(r), m) for r, m, _ in eng.tasks())
+            gc = sorted(tuple(sorted(c)) for c in eng.cliques())
+            st = eng.stats()
+            kids, cl, br = ot.expand(recs, nm, 0, mode)
+            print(f"task {i}: size {len(recs)}, {len(pieces)} pieces, {len(kids)} children, {br} branches", flush=True)
+            assert st.tasks_expanded == 1, i
+            assert st.branches == br, i
             assert gk == sorted((tuple(r), m) for r, m in kids), i
             assert gc == cl, i

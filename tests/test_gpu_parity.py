@@ -36,13 +36,14 @@ def test_golden_cases_bit_exact(dqc, manifest, tmp_path):
 
 
 def test_small_batches_same_result(dqc, manifest, tmp_path):
-    """Scheduling invariance: a tiny batch size (many launches, deep stack) gives the same
-    bytes and counts as the default."""
+    """Scheduling invariance: a tiny batch size (many launches, deep stack) in plain FIFO
+    order gives the same bytes and counts as the default."""
     c = next(x for x in manifest["cases"] if x["name"] == "workload3_g0")
     cfg = dqc.Config()
     cfg.pool_tasks = 7
     cfg.workgroups = 5
     cfg.host_addfail_check = 1
+    cfg.schedule = dqc.SCHED_FIFO
     cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
     assert cnt == c["maximal"] and st.tasks_expanded == c["tasks"] and st.branches == c["branches"]
     assert hashlib.sha256(open(mx, "rb").read()).hexdigest() == c["sha256"]
@@ -92,3 +93,20 @@ def test_serial_level0_same_result(dqc, manifest, tmp_path, monkeypatch):
         assert cnt == c["maximal"] and st.tasks_expanded == c["tasks"] and st.branches == c["branches"], name
         if c["sha256"]:
             assert hashlib.sha256(open(mx, "rb").read()).hexdigest() == c["sha256"], name
+
+
+def test_split_everything_same_result(dqc, manifest, tmp_path):
+    """Every task with 2 or more branches cut into pieces of 3: same bytes and counts."""
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.split_branches = 2
+    cfg.split_chunk = 3
+    bad = []
+    for c in manifest["cases"]:
+        cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
+                or st.raw_cliques != c["raw_cliques"]):
+            bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"], st.branches, c["branches"]))
+    assert not bad, bad
