@@ -118,8 +118,12 @@ struct KParams {
     int32_t split_c;            // least branches per piece of a split task
 };
 
+// Id bitmaps (parent-live, parent-member, child-member, child-candidate, in-L(v),
+// critical): in LDS when the condensed graph has at most KMAX vertices, otherwise in the
+// workgroup's HBM area. Either way they are all-zero between tasks: each branch and
+// task clears the words of the vertices it set.
+constexpr int NBITMAPS = 6;
 struct Shared {
-    uint32_t plive[KW], pmem[KW], cmem[KW], ccand[KW], cL[KW], ccrit[KW];
     unsigned long long sortk[SORT_N];
     int32_t ho[HMAX + 2], hi[HMAX + 2], so[HMAX + 2], si[HMAX + 2];
     int32_t cnt[12];
@@ -156,6 +160,8 @@ struct W {
     Shared* S;
     int tid;
     int kw;  // bitmap words in use
+    uint32_t *plive, *pmem, *cmem, *ccand, *cL, *ccrit;  // id bitmaps (LDS or HBM)
+    int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
     uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
     unsigned long long* hsort;  // HBM sort keys for children above SORT_N (2k entries)
@@ -401,7 +407,7 @@ struct W {
         for (int i = tid; i < nr; i += NT) {
             int w = R[i];
             if (cand_valid(w, cm, ub, lb, meo, mei)) keep[atomicAdd(&S->cnt[0], 1)] = w;
-            else { drop[atomicAdd(&S->cnt[1], 1)] = w; bcl(S->ccand, w); }
+            else { drop[atomicAdd(&S->cnt[1], 1)] = w; bcl(ccand, w); }
         }
         __syncthreads();
         int k = S->cnt[0];
@@ -419,9 +425,9 @@ struct W {
 #pragma unroll
             for (int u = 0; u < 8; ++u) a[u] = nbr[b + u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) c += (uint32_t)bt(S->ccand, a[u]);
+            for (int u = 0; u < 8; ++u) c += (uint32_t)bt(ccand, a[u]);
         }
-        for (; b < e; ++b) c += (uint32_t)bt(S->ccand, nbr[b]);
+        for (; b < e; ++b) c += (uint32_t)bt(ccand, nbr[b]);
         return c;
     }
     // Same count for one wavefront (lanes stride the list), summed across the wave.
@@ -430,10 +436,10 @@ struct W {
         unsigned long long i = b + lane;
         for (; i + 3 * 64 < e; i += 4 * 64) {
             const int a0 = nbr[i], a1 = nbr[i + 64], a2 = nbr[i + 128], a3 = nbr[i + 192];
-            c += (uint32_t)bt(S->ccand, a0) + (uint32_t)bt(S->ccand, a1) + (uint32_t)bt(S->ccand, a2) +
-                 (uint32_t)bt(S->ccand, a3);
+            c += (uint32_t)bt(ccand, a0) + (uint32_t)bt(ccand, a1) + (uint32_t)bt(ccand, a2) +
+                 (uint32_t)bt(ccand, a3);
         }
-        for (; i < e; i += 64) c += (uint32_t)bt(S->ccand, nbr[i]);
+        for (; i < e; i += 64) c += (uint32_t)bt(ccand, nbr[i]);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
         return c;
     }
@@ -475,10 +481,10 @@ struct W {
         for (int i = tid; i < nd; i += NT) {
             int u = D[i];
             thread_scan(P->on, oo[u], oo[u + 1], [&](int w) {
-                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&cic[w], 1u);
+                if (bt(cmem, w) || bt(ccand, w)) atomicSub(&cic[w], 1u);
             });
             thread_scan(P->in, io[u], io[u + 1], [&](int w) {
-                if (bt(S->cmem, w) || bt(S->ccand, w)) atomicSub(&coc[w], 1u);
+                if (bt(cmem, w) || bt(ccand, w)) atomicSub(&coc[w], 1u);
             });
             bytes += 4ull * ((oo[u + 1] - oo[u]) + (io[u + 1] - io[u])) + 32;
         }
@@ -722,7 +728,7 @@ struct W {
         if (tot < P->min_size) return 0;
         const uint32_t mo = (uint32_t)gmd(nm + 1, P->mdo), mi = (uint32_t)gmd(nm + 1, P->mdi);
         const int p = pids[tot], last = pids[tot - 1];
-        if (tid == 0) bcl(S->plive, p);
+        if (tid == 0) bcl(plive, p);
         __syncthreads();
         // The out-, in- and two-hop lists touch disjoint fields (in_can, out_can, lvl2adj),
         // so they are walked in one phase. The reference stops after the out-list when a
@@ -730,21 +736,21 @@ struct W {
         // is unobservable because a failure ends the task.
         int fail = 0, skip = 0;
         wg_scan(P->on, P->oo[p], P->oo[p + 1], [&](int w) {
-            if (bt(S->plive, w)) {
+            if (bt(plive, w)) {
                 uint32_t ic = pic[w] - 1u;
                 pic[w] = ic;
-                if (pim[w] + ic < mi) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
+                if (pim[w] + ic < mi) { if (bt(pmem, w)) fail = 1; else if (w == last) skip = 1; }
             }
         });
         wg_scan(P->in, P->io[p], P->io[p + 1], [&](int w) {
-            if (bt(S->plive, w)) {
+            if (bt(plive, w)) {
                 uint32_t oc = poc[w] - 1u;
                 poc[w] = oc;
-                if (pom[w] + oc < mo) { if (bt(S->pmem, w)) fail = 1; else if (w == last) skip = 1; }
+                if (pom[w] + oc < mo) { if (bt(pmem, w)) fail = 1; else if (w == last) skip = 1; }
             }
         });
         wg_scan(P->tn, P->to[p], P->to[p + 1], [&](int w) {
-            if (bt(S->plive, w) && !bt(S->pmem, w)) pl2[w] -= 1u;
+            if (bt(plive, w) && !bt(pmem, w)) pl2[w] -= 1u;
         });
         if (tid == 0)
             bytes += 4ull * ((P->oo[p + 1] - P->oo[p]) + (P->io[p + 1] - P->io[p]) + (P->to[p + 1] - P->to[p])) + 48;
@@ -782,40 +788,57 @@ struct W {
         }
         begin_child(cm);
         // members: the parent's plus v
+        dirty_m = nm + 1;
         for (int a = tid; a <= nm; a += NT) {
             int x = a < nm ? pids[a] : v;
             mem[a] = x;
             com[x] = pom[x]; coc[x] = poc[x]; cim[x] = pim[x]; cic[x] = pic[x];
-            bs(S->cmem, x);
+            bs(cmem, x);
         }
         // candidates within two hops of v (the others fail diameter pruning)
         wg_scan(P->tn, P->to[v], P->to[v + 1], [&](int w) {
-            if (bt(S->plive, w) && !bt(S->pmem, w) && w != v) {
+            if (bt(plive, w) && !bt(pmem, w) && w != v) {
                 com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
-                bs(S->cL, w);
+                bs(cL, w);
                 lst[atomicAdd(&S->cnt[3], 1)] = w;
             }
         });
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
         __syncthreads();
         const int nl = S->cnt[3];
+        dirty_l = nl;
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
         wg_scan(P->on, P->oo[v], P->oo[v + 1], [&](int w) {
-            if (bt(S->cmem, w) || bt(S->cL, w)) { cim[w] += 1u; cic[w] -= 1u; }
+            if (bt(cmem, w) || bt(cL, w)) { cim[w] += 1u; cic[w] -= 1u; }
         });
         wg_scan(P->in, P->io[v], P->io[v + 1], [&](int w) {
-            if (bt(S->cmem, w) || bt(S->cL, w)) { com[w] += 1u; coc[w] -= 1u; }
+            if (bt(cmem, w) || bt(cL, w)) { com[w] += 1u; coc[w] -= 1u; }
         });
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
-        return branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1) >= 0;
+        const long long t = branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1);
+        end_child();
+        return t >= 0;
     }
 
     __device__ __forceinline__ void begin_child(int cm) {
-        for (int w = tid; w < kw; w += NT) { S->cmem[w] = 0; S->ccand[w] = 0; S->cL[w] = 0; S->ccrit[w] = 0; }
         if (tid == 0) { S->cnt[3] = 0; S->cnt[4] = 0; }
         clear_hist(cm);
         __syncthreads();
+    }
+    // Clears the child's bitmap words (every bit set in a branch belongs to one of its
+    // members or L(v) candidates), so the bitmaps are all-zero between branches, also
+    // across launches when they live in HBM.
+    __device__ __forceinline__ void end_child() {
+        __syncthreads();
+        for (int i = tid; i < dirty_m + dirty_l; i += NT) {
+            const int x = i < dirty_m ? mem[i] : lst[i - dirty_m];
+            const int wd = x >> 5;
+            cmem[wd] = 0; ccand[wd] = 0; cL[wd] = 0; ccrit[wd] = 0;
+        }
+        __syncthreads();
+        dirty_m = 0;
+        dirty_l = 0;
     }
 
     // Branch s of the level-0 task, built without replaying the s earlier removals.
@@ -827,6 +850,7 @@ struct W {
     __device__ __forceinline__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
         const int v = k - s - 1, tot = k - s;
         begin_child(1);
+        dirty_m = 1;
         const unsigned long long *oo = P->oo, *io = P->io;
         if (tid == 0) {
             mem[0] = v;
@@ -834,7 +858,7 @@ struct W {
             coc[v] = cnt_lt(P->on, oo[v], oo[v + 1], v);
             cic[v] = cnt_lt(P->in, io[v], io[v + 1], v);
             pl2[v] = root[v].lvl2adj - ((uint32_t)(P->lto[v + 1] - P->lto[v]) - cnt_lt(P->ltn, P->lto[v], P->lto[v + 1], tot));
-            bs(S->cmem, v);
+            bs(cmem, v);
         }
         for (unsigned long long e = P->to[v] + tid; e < P->to[v + 1]; e += NT) {
             int u = P->tn[e];
@@ -845,14 +869,17 @@ struct W {
             cic[u] = cnt_lt(P->in, io[u], io[u + 1], v);
             const unsigned long long l0 = P->lto[u], l1 = P->lto[u + 1];
             pl2[u] = root[u].lvl2adj - ((uint32_t)(l1 - l0) - cnt_lt(P->ltn, l0, l1, tot));
-            bs(S->cL, u);
+            bs(cL, u);
             lst[atomicAdd(&S->cnt[3], 1)] = u;
         }
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
         __syncthreads();
         const int nl = S->cnt[3];
+        dirty_l = nl;
         __syncthreads();
-        return branch_tail(0, tot, nl, MODE_HOST_L0, MODE_DEVICE, s);
+        const long long t = branch_tail(0, tot, nl, MODE_HOST_L0, MODE_DEVICE, s);
+        end_child();
+        return t;
     }
 
     // Diameter test through child write (reference src/device_kernels.cu:122-165 /
@@ -866,7 +893,7 @@ struct W {
             for (int i = tid; i < nl; i += NT) {
                 int w = lst[i];
                 if (com[w] + coc[w] >= mo2 && cim[w] + cic[w] >= mi2) {
-                    bs(S->ccand, w);
+                    bs(ccand, w);
                     la[atomicAdd(&S->cnt[4], 1)] = w;
                     atomicAdd(&S->ho[com[w]], 1);
                     atomicAdd(&S->hi[cim[w]], 1);
@@ -909,13 +936,13 @@ struct W {
             int x = mem[a];
             if (com[x] + coc[x] == tho && coc[x] > 0u) {
                 wg_scan(P->on, P->oo[x], P->oo[x + 1], [&](int w) {
-                    if (bt(S->ccand, w)) bs(S->ccrit, w);
+                    if (bt(ccand, w)) bs(ccrit, w);
                 });
                 if (tid == 0) bytes += 4ull * (P->oo[x + 1] - P->oo[x]) + 16;
             }
             if (cim[x] + cic[x] == thi && cic[x] > 0u) {
                 wg_scan(P->in, P->io[x], P->io[x + 1], [&](int w) {
-                    if (bt(S->ccand, w)) bs(S->ccrit, w);
+                    if (bt(ccand, w)) bs(ccrit, w);
                 });
                 if (tid == 0) bytes += 4ull * (P->io[x + 1] - P->io[x]) + 16;
             }
@@ -926,7 +953,7 @@ struct W {
         int32_t* rest = (R == la) ? lb : la;
         for (int i = tid; i < nr; i += NT) {
             int w = R[i];
-            if (bt(S->ccrit, w)) K[atomicAdd(&S->cnt[5], 1)] = w;
+            if (bt(ccrit, w)) K[atomicAdd(&S->cnt[5], 1)] = w;
             else rest[atomicAdd(&S->cnt[6], 1)] = w;
         }
         __syncthreads();
@@ -945,7 +972,7 @@ struct W {
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
             wg_scan(P->tn, P->to[c], P->to[c + 1], [&](int w) {
-                if (bt(S->cmem, w) || bt(S->ccand, w)) adjc[w] += 1u;
+                if (bt(cmem, w) || bt(ccand, w)) adjc[w] += 1u;
             });
             if (tid == 0) bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
             __syncthreads();
@@ -958,21 +985,22 @@ struct W {
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
             wg_scan(P->on, P->oo[c], P->oo[c + 1], [&](int w) {
-                if (bt(S->cmem, w) || bt(S->ccand, w)) { cim[w] += 1u; cic[w] -= 1u; }
+                if (bt(cmem, w) || bt(ccand, w)) { cim[w] += 1u; cic[w] -= 1u; }
             });
             wg_scan(P->in, P->io[c], P->io[c + 1], [&](int w) {
-                if (bt(S->cmem, w) || bt(S->ccand, w)) { com[w] += 1u; coc[w] -= 1u; }
+                if (bt(cmem, w) || bt(ccand, w)) { com[w] += 1u; coc[w] -= 1u; }
             });
             if (tid == 0) bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
             __syncthreads();
         }
         for (int i = tid; i < nk; i += NT) {
             int c = K[i];
-            bs(S->cmem, c);
-            bcl(S->ccand, c);
+            bs(cmem, c);
+            bcl(ccand, c);
             mem[cm + i] = c;
         }
         cm += nk;
+        dirty_m = cm;
         clear_hist(cm);
         if (tid == 0) S->cnt[7] = 0;
         __syncthreads();
@@ -986,7 +1014,7 @@ struct W {
                 atomicAdd(&S->ho[com[w]], 1);
                 atomicAdd(&S->hi[cim[w]], 1);
             } else {
-                bcl(S->ccand, w);
+                bcl(ccand, w);
             }
         }
         int nkeep = read_counter(7);
@@ -1015,8 +1043,8 @@ struct W {
             int x = (int)r.id;
             pids[i] = x;
             pom[x] = r.out_mem; poc[x] = r.out_can; pim[x] = r.in_mem; pic[x] = r.in_can; pl2[x] = r.lvl2adj;
-            bs(S->plive, x);
-            if (i < nm) bs(S->pmem, x);
+            bs(plive, x);
+            if (i < nm) bs(pmem, x);
         }
         bytes += 24ull * T / NT + 1;
         if (tid == 0 && !piece) atomicAdd(&P->g->tasks, 1ull);
@@ -1051,7 +1079,10 @@ struct W {
             tick(0);
             if (wrote) first_host_child = false;
         }
-        for (int w = tid; w < kw; w += NT) { S->plive[w] = 0; S->pmem[w] = 0; }
+        for (int i = tid; i < T; i += NT) {
+            const int wd = pids[i] >> 5;
+            plive[wd] = 0; pmem[wd] = 0;
+        }
         __syncthreads();
     }
 };
@@ -1060,6 +1091,7 @@ struct W {
 #ifndef DQC_WAVES_PER_EU
 #define DQC_WAVES_PER_EU 8
 #endif
+template <bool BIG>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DQC_WAVES_PER_EU)))
 void expand_kernel(KParams P) {
     __shared__ Shared S;
@@ -1079,7 +1111,18 @@ void expand_kernel(KParams P) {
     w.pids = lists; w.mem = lists + k; w.la = lists + 2 * k; w.lb = lists + 3 * k; w.lc = lists + 4 * k;
     w.lst = lists + 5 * k; w.crit = lists + 6 * k; w.gat = lists + 7 * k;
     w.hsort = (unsigned long long*)(lists + 8 * k);
-    for (int i = threadIdx.x; i < w.kw; i += NT) { S.plive[i] = 0; S.pmem[i] = 0; }
+    uint32_t* bm;
+    if constexpr (BIG) {
+        bm = base + 22 * k;  // kept all-zero between tasks; zeroed when the graph is uploaded
+    } else {
+        __shared__ uint32_t lds_bm[NBITMAPS * KW];
+        bm = lds_bm;
+        for (int i = threadIdx.x; i < NBITMAPS * KW; i += NT) lds_bm[i] = 0;
+    }
+    const int bw = BIG ? w.kw : KW;
+    w.plive = bm; w.pmem = bm + bw; w.cmem = bm + 2 * bw; w.ccand = bm + 3 * bw; w.cL = bm + 4 * bw; w.ccrit = bm + 5 * bw;
+    w.dirty_m = 0;
+    w.dirty_l = 0;
     if (threadIdx.x == 0) S.acc_bytes = 0;
     __syncthreads();
     while (true) {
@@ -1169,6 +1212,8 @@ struct dqc_handle {
     std::string err;
     // graph
     int32_t n = 0;
+    bool big = false;    // condensed graph above KMAX vertices: HBM id bitmaps
+    int launch_wg = 0;   // workgroups per launch (the scratch bounds it for large graphs)
     unsigned long long *d_oo = nullptr, *d_io = nullptr, *d_to = nullptr;
     int32_t *d_on = nullptr, *d_in = nullptr, *d_tn = nullptr;
     int32_t *d_mdo = nullptr, *d_mdi = nullptr, *d_ubo = nullptr, *d_ubi = nullptr;
@@ -1269,7 +1314,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!hip_ok(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device), "device query"))
         return DQC_E_HIP;
     int per_cu = 0;
-    if (!hip_ok(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expand_kernel, NT, 0), "occupancy"))
+    if (!hip_ok(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expand_kernel<false>, NT, 0), "occupancy"))
         return DQC_E_HIP;
     if (per_cu < 1) per_cu = 1;
     h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
@@ -1311,14 +1356,12 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
                             const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
                             const int32_t* two_nbr) {
     if (!h || n < 0) return DQC_E_ARG;
-    if (n > KMAX) {
-        h->err = "condensed graph has " + std::to_string(n) + " vertices; this build handles at most " +
-                 std::to_string(KMAX);
-        return DQC_E_UNSUPPORTED;
-    }
     hipSetDevice(h->device);
     free_graph(h);
     h->n = n;
+    // id bitmaps no longer fit LDS: the kernel variant keeps them in HBM (DQC_HBM_BITMAPS=1
+    // selects it for any size, which the tests use to run it on small graphs)
+    h->big = n > KMAX || std::getenv("DQC_HBM_BITMAPS") != nullptr;
     size_t m_out = out_off[n], m_in = in_off[n], m_two = two_off[n];
     if (!dev_alloc(h, &h->d_oo, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_io, (size_t)n + 1, "graph alloc") ||
         !dev_alloc(h, &h->d_to, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_on, m_out, "graph alloc") ||
@@ -1347,10 +1390,19 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
             (m_two && !hip_ok(h, hipMemcpy(h->d_ltn, ltn.data(), sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload")))
             return DQC_E_HIP;
     }
-    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each, and
-    // the HBM sort area (2k 64-bit keys: a child has at most k vertices, padded to a power of 2)
-    h->stride = 22ull * (unsigned long long)std::max(n, 1);
-    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(h->workgroups + 1), "scratch alloc")) return DQC_E_HIP;
+    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each, the
+    // HBM sort area (2k 64-bit keys: a child has at most k vertices, padded to a power of
+    // 2) and, for large graphs, the six id bitmaps (all words even, so 8-byte aligned)
+    const unsigned long long kk = (unsigned long long)std::max(n, 1), kwords = (kk + 31) / 32;
+    h->stride = 22ull * kk + (h->big ? (unsigned long long)NBITMAPS * kwords : 0ull);
+    h->stride += h->stride & 1ull;
+    // keep the scratch within 96 GiB (fewer resident workgroups for very large graphs)
+    const unsigned long long cap_wg = (96ull << 30) / (h->stride * sizeof(uint32_t));
+    h->launch_wg = (int)std::max<unsigned long long>(1, std::min<unsigned long long>(h->workgroups, cap_wg > 1 ? cap_wg - 1 : 1));
+    h->st.workgroups = h->launch_wg;
+    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(h->launch_wg + 1), "scratch alloc")) return DQC_E_HIP;
+    if (h->big && !hip_ok(h, hipMemset(h->d_scratch, 0, sizeof(uint32_t) * h->stride * (size_t)(h->launch_wg + 1)), "scratch clear"))
+        return DQC_E_HIP;
     h->st.n_condensed = n;
     return DQC_OK;
 }
@@ -1457,6 +1509,10 @@ dqc_status dqc_reset(dqc_handle* h) {
         return DQC_E_HIP;
     h->extra_tasks = 0;
     h->st.tasks_expanded = h->st.branches = h->st.raw_cliques = 0;
+    // a failed launch can leave id-bitmap bits set in the HBM variant
+    if (h->big && h->d_scratch &&
+        !hip_ok(h, hipMemset(h->d_scratch, 0, sizeof(uint32_t) * h->stride * (size_t)(h->launch_wg + 1)), "reset"))
+        return DQC_E_HIP;
     return DQC_OK;
 }
 
@@ -1586,9 +1642,10 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
     q.root_first = ~0ull;
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
-    const int grid = (int)std::min<unsigned long long>((unsigned long long)h->workgroups, ntask);
+    const int grid = (int)std::min<unsigned long long>((unsigned long long)h->launch_wg, ntask);
     hipEventRecord(h->ev0, h->stream);
-    expand_kernel<<<grid, NT, 0, h->stream>>>(P);
+    if (h->big) expand_kernel<true><<<grid, NT, 0, h->stream>>>(P);
+    else expand_kernel<false><<<grid, NT, 0, h->stream>>>(P);
     hipEventRecord(h->ev1, h->stream);
     if (!hip_ok(h, hipGetLastError(), "expand launch")) return DQC_E_HIP;
     GCtr g{};

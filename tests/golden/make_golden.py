@@ -35,6 +35,20 @@ def write_random(path, rng, n, p):
             f.write(" ".join(str(v) for v in range(n) if v != u and rng.random() < p) + "\n")
 
 
+def write_reciprocal(path, rng, n, arcs_per_vertex, back):
+    """n vertices, n * arcs_per_vertex random arcs, each reversed too with probability back."""
+    adj = [set() for _ in range(n)]
+    for _ in range(n * arcs_per_vertex):
+        a, b = rng.randrange(n), rng.randrange(n)
+        if a != b:
+            adj[a].add(b)
+            if rng.random() < back:
+                adj[b].add(a)
+    with open(path, "w") as f:
+        for u in range(n):
+            f.write(" ".join(str(v) for v in sorted(adj[u])) + "\n")
+
+
 def oracle_case(name, path, go, gi, mn, cap=3.0):
     exp = os.path.join(EXPECTED, name + ".max")
     if os.path.exists(exp):
@@ -85,9 +99,19 @@ def main():
         c = oracle_case(name, path, g, gi, mn, cap=2.0)
         if c is None or c["oracle_s"] > 1.0 or c["tasks"] < 2:
             os.remove(path)
+            exp = os.path.join(EXPECTED, name + ".max")
+            if os.path.exists(exp):
+                os.remove(exp)
             continue
         cases.append(c)
         kept += 1
+    # a sparse graph that condenses to more than 16384 vertices: the kernels then keep
+    # their id bitmaps in HBM instead of LDS
+    path = os.path.join(GRAPHS, "bigk20000.txt")
+    write_reciprocal(path, random.Random(7), 20000, 4, 0.7)
+    c = oracle_case("bigk20000", path, 0.6, 0.6, 4, cap=60.0)
+    assert c and c["n_condensed"] > 16384, c
+    cases.append(c)
     cases = [c for c in cases if c]
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump({"workload1_sha256": sha(c1), "cases": cases}, f, indent=1)

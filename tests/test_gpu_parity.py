@@ -110,3 +110,18 @@ def test_split_everything_same_result(dqc, manifest, tmp_path):
                 or st.raw_cliques != c["raw_cliques"]):
             bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"], st.branches, c["branches"]))
     assert not bad, bad
+
+
+def test_hbm_bitmaps_same_result(dqc, manifest, tmp_path, monkeypatch):
+    """The kernel variant for condensed graphs above 16384 vertices keeps its id bitmaps in
+    HBM; forced on every golden case, it gives the same bytes and counts."""
+    monkeypatch.setenv("DQC_HBM_BITMAPS", "1")
+    bad = []
+    for c in manifest["cases"]:
+        cnt, st, mx = run_case(dqc, c, tmp_path)
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
+                or st.raw_cliques != c["raw_cliques"]):
+            bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"]))
+    assert not bad, bad
