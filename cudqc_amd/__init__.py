@@ -19,8 +19,22 @@ EXPORTS = [
     "dqc_open", "dqc_close", "dqc_last_error", "dqc_upload_graph", "dqc_set_tables", "dqc_push_root",
     "dqc_run", "dqc_fetch_cliques", "dqc_get_stats", "dqc_search_file", "dqc_remove_non_max", "dqc_generate",
     "dqc_load_graph", "dqc_level0", "dqc_free", "dqc_step", "dqc_get_phases", "dqc_prepare_file",
-    "dqc_push_task", "dqc_fetch_tasks", "dqc_reset",
+    "dqc_push_task", "dqc_fetch_tasks", "dqc_reset", "dqc_set_task_log", "dqc_fetch_task_log",
+    "dqc_task_features", "dqc_mlp_forward", "dqc_mlp_gradient", "dqc_mlp_train", "dqc_set_cost_model",
+    "dqc_upload_graph_batch", "dqc_push_graph_root", "dqc_fetch_batch_cliques", "dqc_prepare_batch",
+    "dqc_search_batch",
 ]
+
+MLP_IN, MLP_HIDDEN = 7, 64
+MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+
+
+class TaskLog(ctypes.Structure):
+    """dqc_task_log: one expanded task (or piece of a split task)."""
+    _fields_ = [("uid", ctypes.c_uint32), ("parent", ctypes.c_uint32), ("depth", ctypes.c_int32),
+                ("nm", ctypes.c_int32), ("tot", ctypes.c_int32), ("dmin_out", ctypes.c_uint16),
+                ("dmax_out", ctypes.c_uint16), ("dmin_in", ctypes.c_uint16), ("dmax_in", ctypes.c_uint16),
+                ("branches", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 class Config(ctypes.Structure):
@@ -107,12 +121,105 @@ def load():
     lib.dqc_fetch_cliques.restype = ctypes.c_int32
     lib.dqc_fetch_cliques.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                       P32, P32]
+    PF = ctypes.POINTER(ctypes.c_float)
+    PD = ctypes.POINTER(ctypes.c_double)
+    PI64 = ctypes.POINTER(ctypes.c_int64)
+    lib.dqc_set_task_log.restype = ctypes.c_int32
+    lib.dqc_set_task_log.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+    lib.dqc_fetch_task_log.restype = ctypes.c_int32
+    lib.dqc_fetch_task_log.argtypes = [ctypes.c_void_p, PI64, PI64, ctypes.POINTER(TaskLog)]
+    lib.dqc_task_features.restype = ctypes.c_int32
+    lib.dqc_task_features.argtypes = [ctypes.POINTER(TaskLog), ctypes.c_int64, PF]
+    lib.dqc_mlp_forward.restype = ctypes.c_int32
+    lib.dqc_mlp_forward.argtypes = [PF, PF, ctypes.c_int64, PF]
+    lib.dqc_mlp_gradient.restype = ctypes.c_int32
+    lib.dqc_mlp_gradient.argtypes = [PF, PF, PF, ctypes.c_int64, PF, PD]
+    lib.dqc_mlp_train.restype = ctypes.c_int32
+    lib.dqc_mlp_train.argtypes = [PF, PF, PF, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                  ctypes.c_uint64, PD]
+    lib.dqc_set_cost_model.restype = ctypes.c_int32
+    lib.dqc_set_cost_model.argtypes = [ctypes.c_void_p, PF, ctypes.c_int64]
+    PPC = ctypes.POINTER(ctypes.c_char_p)
+    lib.dqc_prepare_batch.restype = ctypes.c_int32
+    lib.dqc_prepare_batch.argtypes = [ctypes.c_void_p, PPC, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int32, P32]
+    lib.dqc_search_batch.restype = ctypes.c_int32
+    lib.dqc_search_batch.argtypes = [PPC, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, PPC,
+                                     ctypes.POINTER(Config), PI64, ctypes.POINTER(Stats)]
+    lib.dqc_fetch_batch_cliques.restype = ctypes.c_int32
+    lib.dqc_fetch_batch_cliques.argtypes = [ctypes.c_void_p, PI64, PI64, P32, P32, P32]
     _lib = lib
     return lib
 
 
 def _b(p):
     return None if p is None else os.fsencode(p)
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def task_features(records):
+    """7 features per TaskLog record (numpy float32 array n x 7)."""
+    import numpy as np
+    n = len(records)
+    arr = (TaskLog * max(n, 1))(*records)
+    x = np.zeros((max(n, 1), MLP_IN), dtype=np.float32)
+    rc = load().dqc_task_features(arr, n, _fp(x))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_task_features failed: {STATUS.get(rc, rc)}")
+    return x[:n]
+
+
+def mlp_forward(params, x):
+    """Cost-model forward pass on the GPU: predictions for the rows of x (n x 7)."""
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(max(len(xx), 1), dtype=np.float32)
+    rc = load().dqc_mlp_forward(_fp(p), _fp(xx), len(xx), _fp(y))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_mlp_forward failed: {STATUS.get(rc, rc)}")
+    return y[:len(xx)]
+
+
+def mlp_gradient(params, x, t):
+    """Gradient (and value) of the mean squared error on the GPU."""
+    import numpy as np
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = ctypes.c_double()
+    rc = load().dqc_mlp_gradient(_fp(p), _fp(xx), _fp(tt), len(xx), _fp(g), ctypes.byref(loss))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_mlp_gradient failed: {STATUS.get(rc, rc)}")
+    return g, loss.value
+
+
+def mlp_train(params, x, t, epochs=20, batch=1024, lr=1e-3, seed=1):
+    """Adam training on the GPU; returns (new params, final mean squared error)."""
+    import numpy as np
+    p = np.array(params, dtype=np.float32, copy=True)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    loss = ctypes.c_double()
+    rc = load().dqc_mlp_train(_fp(p), _fp(xx), _fp(tt), len(xx), int(epochs), int(batch), float(lr), int(seed),
+                              ctypes.byref(loss))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_mlp_train failed: {STATUS.get(rc, rc)}")
+    return p, loss.value
+
+
+def mlp_init(seed=0):
+    """He-initialised parameters (W1, b1, W2, b2, W3, b3 flattened)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    h, i = MLP_HIDDEN, MLP_IN
+    parts = [rng.normal(0, np.sqrt(2.0 / i), h * i), np.zeros(h), rng.normal(0, np.sqrt(2.0 / h), h * h),
+             np.zeros(h), rng.normal(0, np.sqrt(1.0 / h), h), np.zeros(1)]
+    return np.concatenate(parts).astype(np.float32)
 
 
 def search_file(graph_path, out_gamma, in_gamma, min_size, raw_path=None, max_path=None, config=None):
@@ -127,6 +234,25 @@ def search_file(graph_path, out_gamma, in_gamma, min_size, raw_path=None, max_pa
     if rc != DQC_OK:
         raise DQCError(f"dqc_search_file failed: {STATUS.get(rc, rc)}")
     return cnt.value, st
+
+
+def _paths(paths):
+    return (ctypes.c_char_p * max(len(paths), 1))(*[_b(p) for p in paths])
+
+
+def search_batch(graph_paths, out_gamma, in_gamma, min_size, max_paths=None, config=None):
+    """GPU enumeration of several graphs in one handle (batched form). Returns (list of
+    maximal counts per graph, Stats)."""
+    lib = load()
+    n = len(graph_paths)
+    cnt = (ctypes.c_int64 * max(n, 1))()
+    st = Stats()
+    mp = _paths(max_paths) if max_paths is not None else None
+    rc = lib.dqc_search_batch(_paths(graph_paths), n, float(out_gamma), float(in_gamma), int(min_size), mp,
+                              ctypes.byref(config) if config is not None else None, cnt, ctypes.byref(st))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_search_batch failed: {STATUS.get(rc, rc)}")
+    return list(cnt[:n]), st
 
 
 class Engine:
@@ -192,15 +318,62 @@ class Engine:
         """The task stack as a list of (records, nm, mode)."""
         nt, nr = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
-        meta = (ctypes.c_int32 * max(4 * nt.value, 1))()
+        meta = (ctypes.c_int32 * max(8 * nt.value, 1))()
         rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
         out = []
         for i in range(nt.value):
-            off, tot, nm, mode = meta[4 * i:4 * i + 4]
+            off, tot, nm, mode = meta[8 * i:8 * i + 4]
             recs = [tuple(rec[6 * (off + j):6 * (off + j) + 6]) for j in range(tot)]
             out.append((recs, nm, mode))
         return out
+
+    def prepare_batch(self, graph_paths, out_gamma, in_gamma, min_size):
+        n = len(graph_paths)
+        k = (ctypes.c_int32 * max(n, 1))()
+        self._check(self.lib.dqc_prepare_batch(self.h, _paths(graph_paths), n, out_gamma, in_gamma, int(min_size), k),
+                    "dqc_prepare_batch")
+        return list(k[:n])
+
+    def batch_cliques(self):
+        """Raw cliques as (graph, local ids) pairs."""
+        cnt, tot = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_batch_cliques(self.h, ctypes.byref(cnt), ctypes.byref(tot), None, None, None),
+                    "fetch")
+        g = (ctypes.c_int32 * max(cnt.value, 1))()
+        sizes = (ctypes.c_int32 * max(cnt.value, 1))()
+        ids = (ctypes.c_int32 * max(tot.value, 1))()
+        self._check(self.lib.dqc_fetch_batch_cliques(self.h, ctypes.byref(cnt), ctypes.byref(tot), g, sizes, ids), "fetch")
+        out, p = [], 0
+        for i in range(cnt.value):
+            out.append((g[i], list(ids[p:p + sizes[i]])))
+            p += sizes[i]
+        return out
+
+    def set_task_log(self, capacity):
+        self._check(self.lib.dqc_set_task_log(self.h, int(capacity)), "dqc_set_task_log")
+
+    def task_log(self):
+        """The task log as a ctypes array of TaskLog (and the count dropped beyond capacity)."""
+        n, dropped = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_task_log(self.h, ctypes.byref(n), ctypes.byref(dropped), None), "fetch log")
+        arr = (TaskLog * max(n.value, 1))()
+        self._check(self.lib.dqc_fetch_task_log(self.h, ctypes.byref(n), ctypes.byref(dropped), arr), "fetch log")
+        return arr[:n.value], dropped.value
+
+    def task_ids(self):
+        """(uid, parent) of every task on the stack."""
+        nt, nr = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
+        meta = (ctypes.c_int32 * max(8 * nt.value, 1))()
+        rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
+        return [(meta[8 * i + 4] & 0xFFFFFFFF, meta[8 * i + 5] & 0xFFFFFFFF) for i in range(nt.value)]
+
+    def set_cost_model(self, params):
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.dqc_set_cost_model(self.h, _fp(p), p.size), "dqc_set_cost_model")
 
     def close(self):
         if self.h:

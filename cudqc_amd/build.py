@@ -21,7 +21,7 @@ ARCH = "gfx950"
 # packed-fp32 instructions are disabled for all device code built here
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wno-unused-value", "-Wno-unused-result", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include")]
-SOURCES = ["engine.hip", "graph.cpp", "rmnonmax.cpp", "search.cpp", "gen.cpp"]
+SOURCES = ["engine.hip", "costmodel.hip", "graph.cpp", "rmnonmax.cpp", "search.cpp", "gen.cpp"]
 
 
 def _run(cmd):

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "../../include/dqc.h"
+#include "costmodel.hpp"
 #include "graph.hpp"
 
 namespace {
@@ -74,7 +75,21 @@ struct TaskDesc {
     int32_t tot, nm, ncov, mode;
     int32_t iters;  // loop steps to run (pieces); whole tasks run tot - nm - ncov
     int32_t flags;
+    // identity and shape, for the task log and the cost model (pieces keep their task's)
+    uint32_t uid, parent;
+    uint16_t depth, gid;  // gid: graph of a batched run (0 for a single graph)
+    uint16_t dmin_o, dmax_o, dmin_i, dmax_i;  // min / max of out / in degree inside the task
 };
+static_assert(sizeof(TaskDesc) == 56, "task descriptor layout");
+
+// One task-log record per expanded task or piece (the dqc_task_log of the C-ABI).
+struct TaskLog {
+    uint32_t uid, parent;
+    int32_t depth, nm, tot;
+    uint16_t dmin_o, dmax_o, dmin_i, dmax_i;
+    uint32_t branches, flags;
+};
+static_assert(sizeof(TaskLog) == 36, "task log layout");
 
 // Per-queue counters (one per task stack / stream).
 struct QCtr {
@@ -90,6 +105,8 @@ struct GCtr {
     unsigned long long phase[8];   // realtime ticks per phase (profiling runs)
     unsigned int error;
     unsigned int pad;
+    unsigned long long next_uid;   // task ids handed out (the level-0 task is id 0)
+    unsigned long long log_n;      // task-log records requested
 };
 
 struct KParams {
@@ -116,6 +133,10 @@ struct KParams {
     uint32_t* scratch;
     unsigned long long stride;  // uint32 words per workgroup
     int32_t split_c;            // least branches per piece of a split task
+    TaskLog* log;               // task log (null = off)
+    unsigned long long log_cap;
+    int32_t kloc;               // largest vertex count of one graph (scratch arrays, bitmaps)
+    const int32_t* gbase;       // batched graphs: first id of each graph (multiples of 32); null = one graph
 };
 
 // Id bitmaps (parent-live, parent-member, child-member, child-candidate, in-L(v),
@@ -127,15 +148,12 @@ struct Shared {
     unsigned long long sortk[SORT_N];
     int32_t ho[HMAX + 2], hi[HMAX + 2], so[HMAX + 2], si[HMAX + 2];
     int32_t cnt[12];
-    int32_t red[8];
+    int32_t red[12];
     int32_t lu[12];
     unsigned long long bcast;
     unsigned long long acc_bytes;
 };
 
-__device__ __forceinline__ bool bt(const uint32_t* bm, int id) { return (bm[id >> 5] >> (id & 31)) & 1u; }
-__device__ __forceinline__ void bs(uint32_t* bm, int id) { atomicOr(&bm[id >> 5], 1u << (id & 31)); }
-__device__ __forceinline__ void bcl(uint32_t* bm, int id) { atomicAnd(&bm[id >> 5], ~(1u << (id & 31))); }
 
 __device__ __forceinline__ int bitw(unsigned long long x) { return x == 0 ? 0 : 64 - __clzll(x); }
 
@@ -161,7 +179,27 @@ struct W {
     int tid;
     int kw;  // bitmap words in use
     uint32_t *plive, *pmem, *cmem, *ccand, *cL, *ccrit;  // id bitmaps (LDS or HBM)
+    // Batched graphs share one id space; a task's ids all lie in its graph's range
+    // [gb, gb + kloc), and the id-indexed arrays and bitmaps are addressed relative to gb.
+    uint32_t* vbase;   // the workgroup's id-indexed value arrays (graph-relative)
+    int gb;            // first id of the current task's graph (a multiple of 32)
+    uint32_t cur_gid;
+    // graph-relative bit tests on the id bitmaps (gb is a multiple of 32)
+    __device__ __forceinline__ bool bt(const uint32_t* bm, int id) const { return (bm[(id - gb) >> 5] >> (id & 31)) & 1u; }
+    __device__ __forceinline__ void bs(uint32_t* bm, int id) const { atomicOr(&bm[(id - gb) >> 5], 1u << (id & 31)); }
+    __device__ __forceinline__ void bcl(uint32_t* bm, int id) const { atomicAnd(&bm[(id - gb) >> 5], ~(1u << (id & 31))); }
+    // points the id-indexed arrays at graph `gid` of a batched run
+    __device__ __forceinline__ void set_graph(uint32_t gid) {
+        cur_gid = gid;
+        gb = P->gbase ? P->gbase[gid] : 0;
+        const long long k = P->kloc;
+        uint32_t* b = vbase - gb;
+        pom = b; poc = b + k; pim = b + 2 * k; pic = b + 3 * k; pl2 = b + 4 * k;
+        com = b + 5 * k; coc = b + 6 * k; cim = b + 7 * k; cic = b + 8 * k; adjc = b + 9 * k;
+    }
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
+    uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
+    uint32_t nbranch;             // branches run for the current task / piece
     uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
     unsigned long long* hsort;  // HBM sort keys for children above SORT_N (2k entries)
@@ -583,20 +621,31 @@ struct W {
         const int N = cm + nr;
         // key field widths from this child's maxima
         uint32_t mmx = 0, mcx = 0, l2x = 0;
+        uint32_t omin = UINT_MAX, omax = 0, imin = UINT_MAX, imax = 0;  // degree features
         for (int i = tid; i < N; i += NT) {
             int x = i < cm ? mem[i] : R[i - cm];
             gat[i] = x;
             mmx = max(mmx, min(com[x], cim[x]));
             mcx = max(mcx, min(coc[x], cic[x]));
             l2x = max(l2x, pl2[x]);
+            const uint32_t od = com[x] + coc[x], id = cim[x] + cic[x];
+            omin = min(omin, od); omax = max(omax, od); imin = min(imin, id); imax = max(imax, id);
         }
-        if (tid == 0) { S->red[2] = 0; S->red[3] = 0; S->red[4] = 0; }
+        if (tid == 0) {
+            S->red[2] = 0; S->red[3] = 0; S->red[4] = 0;
+            S->red[5] = -1; S->red[6] = 0; S->red[7] = -1; S->red[8] = 0;
+        }
         __syncthreads();
         atomicMax((unsigned*)&S->red[2], mmx);
         atomicMax((unsigned*)&S->red[3], mcx);
         atomicMax((unsigned*)&S->red[4], l2x);
+        atomicMin((unsigned*)&S->red[5], omin);
+        atomicMax((unsigned*)&S->red[6], omax);
+        atomicMin((unsigned*)&S->red[7], imin);
+        atomicMax((unsigned*)&S->red[8], imax);
         __syncthreads();
         mmx = (uint32_t)S->red[2]; mcx = (uint32_t)S->red[3]; l2x = (uint32_t)S->red[4];
+        omin = (uint32_t)S->red[5]; omax = (uint32_t)S->red[6]; imin = (uint32_t)S->red[7]; imax = (uint32_t)S->red[8];
         __syncthreads();
         const uint32_t idx_max = (uint32_t)(P->n - 1);
         const int bi = bitw(idx_max), bl = bitw(l2x), bc = bitw(mcx), bm = bitw(mmx);
@@ -681,6 +730,11 @@ struct W {
             TaskDesc d;
             d.off = o; d.tot = N; d.nm = cm; d.ncov = 0; d.mode = mode;
             d.iters = N - cm; d.flags = 0;
+            d.uid = (uint32_t)(atomicAdd(&P->g->next_uid, 1ull) + 1);
+            d.parent = cur_uid;
+            d.depth = (uint16_t)min(cur_depth + 1u, 65535u); d.gid = (uint16_t)cur_gid;
+            d.dmin_o = (uint16_t)min(omin, 65535u); d.dmax_o = (uint16_t)min(omax, 65535u);
+            d.dmin_i = (uint16_t)min(imin, 65535u); d.dmax_i = (uint16_t)min(imax, 65535u);
             dd[t] = d;
         }
         bytes += 24ull * N / NT + 1;
@@ -690,7 +744,7 @@ struct W {
 
     // Pushes the current state of the task (live vertices in position order with their
     // current degrees and lvl2adj) as a piece that runs `iters` loop steps from here.
-    __device__ __forceinline__ void write_piece(int tot, int nm, int mode, bool remove_first, int iters) {
+    __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters) {
         unsigned long long a = 0;
         if (tid == 0) {
             const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)tot;
@@ -709,7 +763,7 @@ struct W {
             dv[i] = dqc::TaskVertex{(uint32_t)x, pom[x], poc[x], pim[x], pic[x], pl2[x]};
         }
         if (tid == 0) {
-            TaskDesc d;
+            TaskDesc d = src;
             d.off = o; d.tot = tot; d.nm = nm; d.ncov = 0; d.mode = mode;
             d.iters = iters;
             d.flags = TF_PIECE | (remove_first ? TF_REMOVE_FIRST : 0);
@@ -833,7 +887,7 @@ struct W {
         __syncthreads();
         for (int i = tid; i < dirty_m + dirty_l; i += NT) {
             const int x = i < dirty_m ? mem[i] : lst[i - dirty_m];
-            const int wd = x >> 5;
+            const int wd = (x - gb) >> 5;
             cmem[wd] = 0; ccand[wd] = 0; cL[wd] = 0; ccrit[wd] = 0;
         }
         __syncthreads();
@@ -1038,6 +1092,10 @@ struct W {
         const int T = d.tot, nm = d.nm, mode = d.mode;
         const int ncov = mode == MODE_DEVICE ? 0 : d.ncov;
         const bool piece = (d.flags & TF_PIECE) != 0;
+        set_graph(d.gid);
+        cur_uid = d.uid;
+        cur_depth = d.depth;
+        nbranch = 0;
         for (int i = tid; i < T; i += NT) {
             dqc::TaskVertex r = rv[i];
             int x = (int)r.id;
@@ -1059,7 +1117,7 @@ struct W {
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
             if (split && (j - ncov) % chunk == 0)
-                write_piece(tot, nm, mode, j != ncov, min(chunk, expansions - j));
+                write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
             if (j != ncov || remove_first) {
                 tick(0);
                 int r = remove_one(tot, nc, nm);
@@ -1076,11 +1134,22 @@ struct W {
             }
             if (split) continue;
             bool wrote = branch(tot, nm, mode, first_host_child);
+            ++nbranch;
             tick(0);
             if (wrote) first_host_child = false;
         }
+        if (P->log && tid == 0) {
+            const unsigned long long li = atomicAdd(&P->g->log_n, 1ull);
+            if (li < P->log_cap) {
+                TaskLog r;
+                r.uid = d.uid; r.parent = d.parent; r.depth = d.depth; r.nm = nm; r.tot = T;
+                r.dmin_o = d.dmin_o; r.dmax_o = d.dmax_o; r.dmin_i = d.dmin_i; r.dmax_i = d.dmax_i;
+                r.branches = nbranch; r.flags = (uint32_t)d.flags;
+                P->log[li] = r;
+            }
+        }
         for (int i = tid; i < T; i += NT) {
-            const int wd = pids[i] >> 5;
+            const int wd = (pids[i] - gb) >> 5;
             plive[wd] = 0; pmem[wd] = 0;
         }
         __syncthreads();
@@ -1099,14 +1168,14 @@ void expand_kernel(KParams P) {
     w.P = &P;
     w.S = &S;
     w.tid = threadIdx.x;
-    w.kw = (P.n + 31) >> 5;
+    w.kw = (P.kloc + 31) >> 5;
     w.bytes = 0;
     for (int i = 0; i < 8; ++i) w.ph[i] = 0;
     w.ph_t = P.profile ? __builtin_amdgcn_s_memrealtime() : 0;
-    const unsigned long long k = (unsigned long long)P.n;
+    const unsigned long long k = (unsigned long long)P.kloc;
     uint32_t* base = P.scratch + (unsigned long long)blockIdx.x * P.stride;
-    w.pom = base; w.poc = base + k; w.pim = base + 2 * k; w.pic = base + 3 * k; w.pl2 = base + 4 * k;
-    w.com = base + 5 * k; w.coc = base + 6 * k; w.cim = base + 7 * k; w.cic = base + 8 * k; w.adjc = base + 9 * k;
+    w.vbase = base;
+    w.set_graph(0);
     int32_t* lists = (int32_t*)(base + 10 * k);
     w.pids = lists; w.mem = lists + k; w.la = lists + 2 * k; w.lb = lists + 3 * k; w.lc = lists + 4 * k;
     w.lst = lists + 5 * k; w.crit = lists + 6 * k; w.gat = lists + 7 * k;
@@ -1132,6 +1201,9 @@ void expand_kernel(KParams P) {
         if (t >= P.in_ntask) break;
         if (P.split_s) {
             const TaskDesc d = P.in_desc[0];
+            w.set_graph(d.gid);
+            w.cur_uid = d.uid;
+            w.cur_depth = d.depth;
             w.branch_root(P.split_s[t], d.tot, P.in_vtx + d.off);
         } else {
             TaskDesc d = P.in_desc[t];
@@ -1146,6 +1218,22 @@ void expand_kernel(KParams P) {
             w.tick(0);
             for (int i = 0; i < 8; ++i) atomicAdd(&P.g->phase[i], w.ph[i]);
         }
+    }
+}
+
+// Predicted log(1 + subtree branches) of tasks d[0..n) (the learned schedule's estimate);
+// the model is staged in LDS, one task per thread.
+__global__ __launch_bounds__(256) void desc_cost_kernel(const float* __restrict__ params, const TaskDesc* __restrict__ d,
+                                                        unsigned long long n, float* __restrict__ y) {
+    __shared__ float p[dqc_cost::NPARAM];
+    for (int i = threadIdx.x; i < dqc_cost::NPARAM; i += blockDim.x) p[i] = params[i];
+    __syncthreads();
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const TaskDesc t = d[i];
+        float x[dqc_cost::IN];
+        dqc_cost::features(t.nm, t.tot, t.depth, t.dmin_o, t.dmax_o, t.dmin_i, t.dmax_i, x);
+        y[i] = dqc_cost::forward(p, x);
     }
 }
 
@@ -1212,6 +1300,10 @@ struct dqc_handle {
     std::string err;
     // graph
     int32_t n = 0;
+    int32_t kloc = 0;    // largest vertex count of one graph
+    int ngraphs = 0;     // graphs in the id space (batched runs: more than one)
+    int32_t* d_gbase = nullptr;
+    std::vector<int32_t> gbase;
     bool big = false;    // condensed graph above KMAX vertices: HBM id bitmaps
     int launch_wg = 0;   // workgroups per launch (the scratch bounds it for large graphs)
     unsigned long long *d_oo = nullptr, *d_io = nullptr, *d_to = nullptr;
@@ -1247,6 +1339,11 @@ struct dqc_handle {
     unsigned long long phase[8] = {0};
     std::vector<TaskDesc> hbatch;  // scheduled batch descriptors (host copy for the upload)
     std::vector<double> hcost;
+    TaskLog* d_log = nullptr;      // task log (dqc_set_task_log)
+    unsigned long long log_cap = 0;
+    float* d_model = nullptr;      // cost model (dqc_set_cost_model)
+    float* d_cost = nullptr;       // per-task predictions of the current batch
+    std::vector<float> hpred;
     std::vector<void*> allocs;
 };
 
@@ -1269,7 +1366,7 @@ bool dev_alloc(dqc_handle* h, T** p, size_t count, const char* what) {
 
 void free_graph(dqc_handle* h) {
     void* gp[] = {h->d_oo, h->d_io, h->d_to, h->d_on, h->d_in, h->d_tn, h->d_scratch,
-                  h->d_lto, h->d_ltn, h->d_flags, h->d_split};
+                  h->d_lto, h->d_ltn, h->d_flags, h->d_split, h->d_gbase};
     for (void* p : gp)
         if (p) {
             hipFree(p);
@@ -1279,6 +1376,28 @@ void free_graph(dqc_handle* h) {
     h->d_on = h->d_in = h->d_tn = nullptr;
     h->d_scratch = nullptr;
     h->d_lto = nullptr; h->d_ltn = nullptr; h->d_flags = nullptr; h->d_split = nullptr;
+    h->d_gbase = nullptr;
+}
+
+// Descriptor of a task pushed from the host, with the degree features the kernels
+// compute for the children they write.
+TaskDesc host_desc(const uint32_t* records, int32_t nvert, int32_t nm, int32_t ncov, int32_t mode,
+                   unsigned long long off, uint32_t uid, uint32_t depth) {
+    TaskDesc d{};
+    d.off = off;
+    d.tot = nvert; d.nm = nm; d.ncov = ncov; d.mode = mode;
+    d.iters = nvert - nm - ncov; d.flags = 0;
+    d.uid = uid; d.parent = UINT32_MAX; d.depth = (uint16_t)std::min<uint32_t>(depth, 65535u);
+    uint32_t omin = UINT32_MAX, omax = 0, imin = UINT32_MAX, imax = 0;
+    for (int32_t i = 0; i < nvert; ++i) {
+        const uint32_t* r = records + 6 * i;  // id, out_mem, out_can, in_mem, in_can, lvl2adj
+        const uint32_t od = r[1] + r[2], id = r[3] + r[4];
+        omin = std::min(omin, od); omax = std::max(omax, od);
+        imin = std::min(imin, id); imax = std::max(imax, id);
+    }
+    d.dmin_o = (uint16_t)std::min<uint32_t>(omin, 65535u); d.dmax_o = (uint16_t)std::min<uint32_t>(omax, 65535u);
+    d.dmin_i = (uint16_t)std::min<uint32_t>(imin, 65535u); d.dmax_i = (uint16_t)std::min<uint32_t>(imax, 65535u);
+    return d;
 }
 
 }  // namespace
@@ -1352,16 +1471,20 @@ dqc_status dqc_close(dqc_handle* h) {
 
 const char* dqc_last_error(const dqc_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
-dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, const int32_t* out_nbr,
-                            const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
-                            const int32_t* two_nbr) {
-    if (!h || n < 0) return DQC_E_ARG;
+}  // extern "C"
+
+// Uploads n ids' CSR lists; kloc is the largest vertex count of one graph (n itself for
+// a single graph; batched graphs share the id space with 32-aligned bases).
+static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint64_t* out_off, const int32_t* out_nbr,
+                              const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
+                              const int32_t* two_nbr) {
     hipSetDevice(h->device);
     free_graph(h);
     h->n = n;
+    h->kloc = std::max(kloc, 1);
     // id bitmaps no longer fit LDS: the kernel variant keeps them in HBM (DQC_HBM_BITMAPS=1
     // selects it for any size, which the tests use to run it on small graphs)
-    h->big = n > KMAX || std::getenv("DQC_HBM_BITMAPS") != nullptr;
+    h->big = kloc > KMAX || std::getenv("DQC_HBM_BITMAPS") != nullptr;
     size_t m_out = out_off[n], m_in = in_off[n], m_two = two_off[n];
     if (!dev_alloc(h, &h->d_oo, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_io, (size_t)n + 1, "graph alloc") ||
         !dev_alloc(h, &h->d_to, (size_t)n + 1, "graph alloc") || !dev_alloc(h, &h->d_on, m_out, "graph alloc") ||
@@ -1393,7 +1516,7 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
     // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each, the
     // HBM sort area (2k 64-bit keys: a child has at most k vertices, padded to a power of
     // 2) and, for large graphs, the six id bitmaps (all words even, so 8-byte aligned)
-    const unsigned long long kk = (unsigned long long)std::max(n, 1), kwords = (kk + 31) / 32;
+    const unsigned long long kk = (unsigned long long)h->kloc, kwords = (kk + 31) / 32;
     h->stride = 22ull * kk + (h->big ? (unsigned long long)NBITMAPS * kwords : 0ull);
     h->stride += h->stride & 1ull;
     // keep the scratch within 96 GiB (fewer resident workgroups for very large graphs)
@@ -1404,6 +1527,65 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, c
     if (h->big && !hip_ok(h, hipMemset(h->d_scratch, 0, sizeof(uint32_t) * h->stride * (size_t)(h->launch_wg + 1)), "scratch clear"))
         return DQC_E_HIP;
     h->st.n_condensed = n;
+    return DQC_OK;
+}
+
+extern "C" {
+
+dqc_status dqc_upload_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, const int32_t* out_nbr,
+                            const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
+                            const int32_t* two_nbr) {
+    if (!h || n < 0 || !out_off || !in_off || !two_off) return DQC_E_ARG;
+    h->ngraphs = 1;
+    h->gbase.assign(1, 0);
+    return upload_impl(h, n, n, out_off, out_nbr, in_off, in_nbr, two_off, two_nbr);
+}
+
+dqc_status dqc_upload_graph_batch(dqc_handle* h, int32_t ngraphs, const int32_t* nverts,
+                                  const uint64_t* const* out_off, const int32_t* const* out_nbr,
+                                  const uint64_t* const* in_off, const int32_t* const* in_nbr,
+                                  const uint64_t* const* two_off, const int32_t* const* two_nbr) {
+    if (!h || ngraphs < 1 || ngraphs > 65535 || !nverts || !out_off || !out_nbr || !in_off || !in_nbr || !two_off ||
+        !two_nbr)
+        return DQC_E_ARG;
+    std::vector<int32_t> base((size_t)ngraphs);
+    long long total = 0;
+    int32_t kloc = 1;
+    for (int32_t g = 0; g < ngraphs; ++g) {
+        if (nverts[g] < 0) return DQC_E_ARG;
+        base[g] = (int32_t)total;
+        total += ((long long)nverts[g] + 31) / 32 * 32;
+        kloc = std::max(kloc, nverts[g]);
+    }
+    if (total >= INT32_MAX) return DQC_E_UNSUPPORTED;
+    const size_t N = (size_t)total;
+    // union CSR: graph g's vertex v is id base[g] + v; padding ids have empty lists
+    std::vector<uint64_t> oo(N + 1, 0), io(N + 1, 0), to(N + 1, 0);
+    std::vector<int32_t> on, in, tn;
+    auto append = [&](const uint64_t* off, const int32_t* nbr, int32_t nv, int32_t b, std::vector<uint64_t>& o,
+                      std::vector<int32_t>& out) {
+        for (int32_t v = 0; v < nv; ++v) {
+            for (uint64_t e = off[v]; e < off[v + 1]; ++e) out.push_back(nbr[e] + b);
+            o[(size_t)b + v + 1] = out.size();
+        }
+    };
+    for (int32_t g = 0; g < ngraphs; ++g) {
+        append(out_off[g], out_nbr[g], nverts[g], base[g], oo, on);
+        append(in_off[g], in_nbr[g], nverts[g], base[g], io, in);
+        append(two_off[g], two_nbr[g], nverts[g], base[g], to, tn);
+    }
+    for (size_t i = 1; i <= N; ++i) {  // padding ids continue the previous offset
+        oo[i] = std::max(oo[i], oo[i - 1]);
+        io[i] = std::max(io[i], io[i - 1]);
+        to[i] = std::max(to[i], to[i - 1]);
+    }
+    dqc_status rc = upload_impl(h, (int32_t)N, kloc, oo.data(), on.data(), io.data(), in.data(), to.data(), tn.data());
+    if (rc != DQC_OK) return rc;
+    h->ngraphs = ngraphs;
+    h->gbase = base;
+    if (!dev_alloc(h, &h->d_gbase, (size_t)ngraphs, "graph alloc") ||
+        !hip_ok(h, hipMemcpy(h->d_gbase, base.data(), sizeof(int32_t) * ngraphs, hipMemcpyHostToDevice), "upload"))
+        return DQC_E_HIP;
     return DQC_OK;
 }
 
@@ -1430,26 +1612,48 @@ dqc_status dqc_set_tables(dqc_handle* h, const int32_t* md_out, const int32_t* m
     return DQC_OK;
 }
 
+static dqc_status push_root_impl(dqc_handle* h, int32_t gid, const uint32_t* records, int32_t nvert, int32_t ncov);
+
 dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t ncov) {
-    if (!h || nvert < 0 || ncov < 0 || ncov > nvert) return DQC_E_ARG;
+    if (!h || h->ngraphs > 1) return DQC_E_ARG;
+    return push_root_impl(h, 0, records, nvert, ncov);
+}
+
+dqc_status dqc_push_graph_root(dqc_handle* h, int32_t gid, const uint32_t* records, int32_t nvert, int32_t ncov) {
+    if (!h || gid < 0 || gid >= h->ngraphs) return DQC_E_ARG;
+    return push_root_impl(h, gid, records, nvert, ncov);
+}
+
+static dqc_status push_root_impl(dqc_handle* h, int32_t gid, const uint32_t* records, int32_t nvert, int32_t ncov) {
+    if (nvert < 0 || ncov < 0 || ncov > nvert || (nvert > 0 && !records)) return DQC_E_ARG;
     if (nvert == 0) return DQC_OK;
-    if (nvert > h->n) return DQC_E_ARG;
+    const int32_t b = h->gbase[gid];
+    const int32_t nloc = (gid + 1 < h->ngraphs ? h->gbase[gid + 1] : h->n) - b;
+    if (nvert > nloc) return DQC_E_ARG;
+    std::vector<uint32_t> shifted;
+    if (b != 0) {  // local ids to the shared id space
+        shifted.assign(records, records + 6 * (size_t)nvert);
+        for (int32_t i = 0; i < nvert; ++i) shifted[6 * (size_t)i] += (uint32_t)b;
+        records = shifted.data();
+    }
+    for (int32_t i = 0; i < nvert; ++i)
+        if (records[6 * (size_t)i] < (uint32_t)b || records[6 * (size_t)i] >= (uint32_t)(b + nloc)) return DQC_E_ARG;
     if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + 1 > h->cfg.spill_tasks ||
         h->st_v + nvert > h->cfg.spill_vertices) {
         h->err = "level-0 task exceeds the batch or stack capacity";
         return DQC_E_CAPACITY;
     }
     hipSetDevice(h->device);
-    TaskDesc d;
-    d.off = h->st_v;
-    d.tot = nvert; d.nm = 0; d.ncov = ncov; d.mode = MODE_HOST_L0;
-    d.iters = nvert - ncov; d.flags = 0;
+    TaskDesc d = host_desc(records, nvert, 0, ncov, MODE_HOST_L0, h->st_v, 0, 0);
+    d.gid = (uint16_t)gid;
     if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
         !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
         return DQC_E_HIP;
     h->st_t += 1;
     h->st_v += nvert;
-    h->root_pending = std::getenv("DQC_SERIAL_ROOT") == nullptr;
+    // one graph: its level-0 task runs branch-parallel; batched graphs run their level-0
+    // tasks side by side, one workgroup each
+    h->root_pending = h->ngraphs == 1 && std::getenv("DQC_SERIAL_ROOT") == nullptr;
     return DQC_OK;
 }
 
@@ -1466,10 +1670,7 @@ dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, 
         return DQC_E_CAPACITY;
     }
     hipSetDevice(h->device);
-    TaskDesc d;
-    d.off = h->st_v;
-    d.tot = nvert; d.nm = nm; d.ncov = ncov; d.mode = mode;
-    d.iters = nvert - nm - ncov; d.flags = 0;
+    const TaskDesc d = host_desc(records, nvert, nm, ncov, mode, h->st_v, 0, 0);
     if (!hip_ok(h, hipMemcpy(h->d_svtx + h->st_v, records, sizeof(dqc::TaskVertex) * nvert, hipMemcpyHostToDevice), "push") ||
         !hip_ok(h, hipMemcpy(h->d_sdesc + h->st_t, &d, sizeof d, hipMemcpyHostToDevice), "push"))
         return DQC_E_HIP;
@@ -1489,10 +1690,9 @@ dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, in
         !hip_ok(h, hipMemcpy(records, h->d_svtx, sizeof(dqc::TaskVertex) * h->st_v, hipMemcpyDeviceToHost), "stack read"))
         return DQC_E_HIP;
     for (unsigned long long i = 0; i < h->st_t; ++i) {
-        meta[4 * i] = (int32_t)d[i].off;
-        meta[4 * i + 1] = d[i].tot;
-        meta[4 * i + 2] = d[i].nm;
-        meta[4 * i + 3] = d[i].mode;
+        int32_t* m = meta + 8 * i;
+        m[0] = (int32_t)d[i].off; m[1] = d[i].tot; m[2] = d[i].nm; m[3] = d[i].mode;
+        m[4] = (int32_t)d[i].uid; m[5] = (int32_t)d[i].parent; m[6] = d[i].depth; m[7] = d[i].flags;
     }
     return DQC_OK;
 }
@@ -1516,7 +1716,8 @@ dqc_status dqc_reset(dqc_handle* h) {
     return DQC_OK;
 }
 
-static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, int32_t sched, double* cost);
+static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsigned long long first, int32_t sched,
+                              double* cost);
 
 // Pops the newest tasks off the stack into the batch buffer (as many as the batch
 // holds); returns the number popped (0 with an error set when none fits).
@@ -1535,13 +1736,30 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
     size_t skip = 0;  // keep the newest tasks whose records fit the batch buffer
     while (skip < m && vend - tail[skip].off > h->cfg.pool_vertices) ++skip;
     if (skip == m) { h->err = "a task does not fit the batch buffer (raise pool_vertices)"; return DQC_E_CAPACITY; }
+    // ... and whose children fit the stack: a task writes at most one child per step, of
+    // at most its own size, so the batch keeps the newest tasks whose worst-case output
+    // fits the space left once they are popped (always at least one task)
+    {
+        size_t keep = m;  // index of the oldest task kept
+        unsigned long long recs = 0, descs = 0;
+        for (size_t i = m; i-- > skip;) {
+            const TaskDesc& d = tail[i];
+            const unsigned long long steps = (unsigned long long)std::max(0, (d.flags & TF_PIECE) ? d.iters : d.tot - d.nm);
+            const unsigned long long free_v = h->cfg.spill_vertices - d.off, free_t = h->cfg.spill_tasks - (h->st_t - m + i);
+            recs += steps * (unsigned long long)d.tot;
+            descs += steps + MAX_PIECES;
+            if (keep < m && (recs > free_v || descs > free_t)) break;
+            keep = i;
+        }
+        skip = keep;
+    }
     const unsigned long long first = h->st_t - m + skip, cnt = m - skip, v0 = tail[skip].off;
     if (sched == DQC_SCHED_FIFO) {
         refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
     } else {
         const TaskDesc* b = tail.data() + skip;
         h->hcost.resize(cnt);
-        dqc_status rc = batch_costs(h, b, cnt, sched, h->hcost.data());
+        dqc_status rc = batch_costs(h, b, cnt, first, sched, h->hcost.data());
         if (rc != DQC_OK) return rc;
         std::vector<uint32_t> order(cnt);
         for (uint32_t i = 0; i < cnt; ++i) order[i] = i;
@@ -1576,12 +1794,30 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
     return DQC_OK;
 }
 
-// Cost estimates of a batch: SIZE uses branches x task size (every branch scans lists
-// of vertices of the task); LEARNED uses the handle's cost model.
-static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, int32_t sched, double* cost) {
+// Cost estimates of a batch (b = host copy of the stack descriptors first .. first+n):
+// SIZE uses branches x task size (every branch scans lists of vertices of the task);
+// LEARNED evaluates the cost model on the device over the same descriptors. A piece
+// gets the share of its task's steps it runs.
+static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsigned long long first, int32_t sched,
+                              double* cost) {
     if (sched == DQC_SCHED_LEARNED) {
-        h->err = "no cost model set (dqc_set_cost_model)";
-        return DQC_E_ARG;
+        if (!h->d_model) {
+            h->err = "no cost model set (dqc_set_cost_model)";
+            return DQC_E_ARG;
+        }
+        const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+        desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc + first, n, h->d_cost);
+        h->hpred.resize(n);
+        if (!hip_ok(h, hipGetLastError(), "cost launch") ||
+            !hip_ok(h, hipMemcpyAsync(h->hpred.data(), h->d_cost, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream), "cost read") ||
+            !hip_ok(h, hipStreamSynchronize(h->stream), "cost kernel"))
+            return DQC_E_HIP;
+        for (size_t i = 0; i < n; ++i) {
+            const double steps = (double)std::max(1, b[i].tot - b[i].nm);
+            const double share = (b[i].flags & TF_PIECE) ? std::min(1.0, (double)b[i].iters / steps) : 1.0;
+            cost[i] = std::expm1((double)std::min(h->hpred[i], 60.f)) * share;
+        }
+        return DQC_OK;
     }
     for (size_t i = 0; i < n; ++i) {
         const double steps = (double)(b[i].tot - b[i].nm);
@@ -1623,6 +1859,10 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.scratch = h->d_scratch;
     P.stride = h->stride;
     P.split_c = h->cfg.split_chunk;
+    P.log = h->d_log;
+    P.log_cap = h->log_cap;
+    P.kloc = h->kloc;
+    P.gbase = h->d_gbase;
     return P;
 }
 
@@ -1769,6 +2009,66 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
     auto t1 = std::chrono::steady_clock::now();
     h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     *finished = h->st_t == 0;
+    return DQC_OK;
+}
+
+dqc_status dqc_set_cost_model(dqc_handle* h, const float* params, int64_t n_params) {
+    if (!h || !params || n_params != dqc_cost::NPARAM) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    if (!h->d_model) {
+        if (!dev_alloc(h, &h->d_model, dqc_cost::NPARAM, "model alloc") ||
+            !dev_alloc(h, &h->d_cost, h->cfg.pool_tasks, "model alloc"))
+            return DQC_E_HIP;
+    }
+    if (!hip_ok(h, hipMemcpy(h->d_model, params, sizeof(float) * dqc_cost::NPARAM, hipMemcpyHostToDevice), "model upload"))
+        return DQC_E_HIP;
+    return DQC_OK;
+}
+
+dqc_status dqc_set_task_log(dqc_handle* h, int64_t capacity) {
+    if (!h || capacity < 0) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    if (h->d_log) {
+        hipFree(h->d_log);
+        h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->d_log), h->allocs.end());
+        h->d_log = nullptr;
+    }
+    h->log_cap = (unsigned long long)capacity;
+    if (capacity > 0 && !dev_alloc(h, &h->d_log, (size_t)capacity, "log alloc")) return DQC_E_HIP;
+    const unsigned long long zero = 0;
+    if (!hip_ok(h, hipMemcpy((char*)h->d_g + offsetof(GCtr, log_n), &zero, sizeof zero, hipMemcpyHostToDevice), "log reset"))
+        return DQC_E_HIP;
+    return DQC_OK;
+}
+
+dqc_status dqc_fetch_task_log(dqc_handle* h, int64_t* count, int64_t* dropped, dqc_task_log* records) {
+    if (!h || !count) return DQC_E_ARG;
+    hipSetDevice(h->device);
+    GCtr g{};
+    if (!hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
+    const unsigned long long n = std::min(g.log_n, h->log_cap);
+    *count = (int64_t)n;
+    if (dropped) *dropped = (int64_t)(g.log_n - n);
+    if (records && n &&
+        !hip_ok(h, hipMemcpy(records, h->d_log, sizeof(TaskLog) * n, hipMemcpyDeviceToHost), "log read"))
+        return DQC_E_HIP;
+    return DQC_OK;
+}
+
+dqc_status dqc_fetch_batch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids, int32_t* graph, int32_t* sizes,
+                                   int32_t* ids) {
+    if (!h || !count || !total_ids) return DQC_E_ARG;
+    dqc_status rc = dqc_fetch_cliques(h, count, total_ids, sizes, ids);
+    if (rc != DQC_OK || !graph || !sizes || !ids) return rc;
+    size_t p = 0;
+    for (int64_t c = 0; c < *count; ++c) {
+        const int32_t len = sizes[c];
+        const int32_t first = len > 0 ? ids[p] : 0;
+        const int32_t g = (int32_t)(std::upper_bound(h->gbase.begin(), h->gbase.end(), first) - h->gbase.begin()) - 1;
+        graph[c] = g;
+        for (int32_t i = 0; i < len; ++i) ids[p + i] -= h->gbase[g];
+        p += (size_t)len;
+    }
     return DQC_OK;
 }
 

@@ -22,6 +22,25 @@ namespace {
 // n + 1 and reads beyond them only in degenerate cases; here the formula continues).
 int table_len(int n, int min_size) { return 2 * (std::max(n, min_size) + 2) + 2; }
 
+// Writes sets in the reference's raw / RemoveNonMax line format: "<len> v1 ... vk \n"
+// (src/host_functions.cu:1222-1237, src/Quick_rmnonmax.cpp:66-69).
+bool write_sets(const char* path, const std::vector<std::vector<int32_t>>& sets) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return false;
+    std::string line;
+    char tmp[16];
+    for (auto& s : sets) {
+        line.clear();
+        int l = std::snprintf(tmp, sizeof tmp, "%d ", (int)s.size());
+        line.append(tmp, l);
+        for (int32_t x : s) { l = std::snprintf(tmp, sizeof tmp, "%d ", x); line.append(tmp, l); }
+        line.push_back('\n');
+        std::fwrite(line.data(), 1, line.size(), f);
+    }
+    std::fclose(f);
+    return true;
+}
+
 struct Timer {
     std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
     double ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
@@ -104,6 +123,99 @@ dqc_status dqc_prepare_file(dqc_handle* h, const char* graph_path, double out_ga
     s = dqc_set_tables(h, mdo.data(), mdi.data(), ubo.data(), ubi.data(), len, min_size);
     if (s != DQC_OK) return s;
     return dqc_push_root(h, (const uint32_t*)l0.task.data(), (int32_t)l0.task.size(), l0.ncov);
+}
+
+dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int32_t ngraphs, double out_gamma,
+                             double in_gamma, int32_t min_size, int32_t* n_condensed) {
+    if (!h || !graph_paths || ngraphs < 1 || ngraphs > 65535 || !(out_gamma >= 0.5 && out_gamma <= 1.0) ||
+        !(in_gamma >= 0.5 && in_gamma <= 1.0) || min_size <= 1)
+        return DQC_E_ARG;
+    std::vector<dqc::Graph> g((size_t)ngraphs);
+    std::vector<dqc::Level0> l0((size_t)ngraphs);
+    std::vector<int> rcs((size_t)ngraphs, 0), lens((size_t)ngraphs, 0);
+    // one graph per thread: load, two-hop lists, level 0 + condense (each graph alone)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        std::string err;
+        if (!graph_paths[i]) { rcs[i] = 2; continue; }
+        rcs[i] = dqc::load_text(graph_paths[i], g[i], err);
+        if (rcs[i]) continue;
+        dqc::build_twohop(g[i], 1);
+        lens[i] = table_len(g[i].n, min_size);
+        auto mdo = dqc::min_degree_table(out_gamma, lens[i]), mdi = dqc::min_degree_table(in_gamma, lens[i]);
+        dqc::level0_condense(g[i], mdo.data(), mdi.data(), min_size, l0[i]);
+    }
+    int len = 2;
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        if (rcs[i]) return rcs[i] == 1 ? DQC_E_IO : DQC_E_ARG;
+        len = std::max(len, lens[i]);
+        if (n_condensed) n_condensed[i] = (int32_t)l0[i].task.size();
+    }
+    std::vector<int32_t> nv((size_t)ngraphs);
+    std::vector<const uint64_t*> oo(ngraphs), io(ngraphs), to(ngraphs);
+    std::vector<const int32_t*> on(ngraphs), in(ngraphs), tn(ngraphs);
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        // a graph with no level-0 survivors keeps no ids (its condensed graph is empty)
+        nv[i] = l0[i].task.empty() ? 0 : g[i].n;
+        oo[i] = g[i].out.off.data(); on[i] = g[i].out.nbr.data();
+        io[i] = g[i].in.off.data(); in[i] = g[i].in.nbr.data();
+        to[i] = g[i].two.off.data(); tn[i] = g[i].two.nbr.data();
+    }
+    dqc_status s = dqc_upload_graph_batch(h, ngraphs, nv.data(), oo.data(), on.data(), io.data(), in.data(), to.data(),
+                                          tn.data());
+    if (s != DQC_OK) return s;
+    auto mdo = dqc::min_degree_table(out_gamma, len), mdi = dqc::min_degree_table(in_gamma, len);
+    auto ubo = dqc::bound_div_table(out_gamma, len), ubi = dqc::bound_div_table(in_gamma, len);
+    s = dqc_set_tables(h, mdo.data(), mdi.data(), ubo.data(), ubi.data(), len, min_size);
+    if (s != DQC_OK) return s;
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        if (l0[i].task.empty()) continue;
+        s = dqc_push_graph_root(h, i, (const uint32_t*)l0[i].task.data(), (int32_t)l0[i].task.size(), l0[i].ncov);
+        if (s != DQC_OK) return s;
+    }
+    return DQC_OK;
+}
+
+dqc_status dqc_search_batch(const char* const* graph_paths, int32_t ngraphs, double out_gamma, double in_gamma,
+                            int32_t min_size, const char* const* max_paths, const dqc_config* cfg,
+                            int64_t* max_cliques, dqc_stats* st) {
+    dqc_handle* h = nullptr;
+    dqc_status s = dqc_open(cfg, &h);
+    auto done = [&](dqc_status code) {
+        if (code != DQC_OK && h) std::fprintf(stderr, "dqc: %s\n", dqc_last_error(h));
+        dqc_close(h);
+        return code;
+    };
+    if (s != DQC_OK) return done(s);
+    std::vector<int32_t> k((size_t)std::max(ngraphs, 1));
+    s = dqc_prepare_batch(h, graph_paths, ngraphs, out_gamma, in_gamma, min_size, k.data());
+    if (s != DQC_OK) return done(s);
+    s = dqc_run(h, DQC_SCHED_DEFAULT);
+    if (s != DQC_OK) return done(s);
+    int64_t count = 0, total = 0;
+    s = dqc_fetch_batch_cliques(h, &count, &total, nullptr, nullptr, nullptr);
+    if (s != DQC_OK) return done(s);
+    std::vector<int32_t> gid((size_t)count), sizes((size_t)count), ids((size_t)total);
+    s = dqc_fetch_batch_cliques(h, &count, &total, gid.data(), sizes.data(), ids.data());
+    if (s != DQC_OK) return done(s);
+    dqc_stats stats{};
+    dqc_get_stats(h, &stats);
+    std::vector<std::vector<std::vector<int32_t>>> sets((size_t)ngraphs);
+    size_t p = 0;
+    for (int64_t c = 0; c < count; ++c) {
+        sets[gid[c]].emplace_back(ids.begin() + p, ids.begin() + p + sizes[c]);
+        p += (size_t)sizes[c];
+    }
+    int64_t all = 0;
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        auto mx = dqc::maximal_sets(sets[i]);
+        if (max_paths && max_paths[i] && !mx.empty() && !write_sets(max_paths[i], mx)) return done(DQC_E_IO);
+        if (max_cliques) max_cliques[i] = (int64_t)mx.size();
+        all += (int64_t)mx.size();
+    }
+    stats.maximal_cliques = all;
+    if (st) *st = stats;
+    return done(DQC_OK);
 }
 
 int64_t dqc_remove_non_max(const char* in_path, const char* out_path) {

@@ -62,6 +62,18 @@ typedef struct {
     int32_t workgroups;
 } dqc_stats;
 
+/* One record per expanded task (or piece of a split task): the task's id, its parent's
+ * id (UINT32_MAX for tasks pushed by the host), depth, members, size, the min / max out-
+ * and in-degree inside the task, and the branches this record's run made. Summing
+ * branches over a task's descendants gives the subtree cost the scheduling model learns.
+ * No counterpart in the reference (its tasks carry no identity). */
+typedef struct {
+    uint32_t uid, parent;
+    int32_t depth, nm, tot;
+    uint16_t dmin_out, dmax_out, dmin_in, dmax_in;
+    uint32_t branches, flags;  /* flags & 1: a piece of a split task */
+} dqc_task_log;
+
 /* Creates a handle on cfg->device (NULL cfg = defaults). */
 dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out);
 dqc_status dqc_close(dqc_handle* h);
@@ -73,6 +85,20 @@ dqc_status dqc_upload_graph(dqc_handle* h, int32_t n,
                             const uint64_t* out_off, const int32_t* out_nbr,
                             const uint64_t* in_off, const int32_t* in_nbr,
                             const uint64_t* two_off, const int32_t* two_nbr);
+
+/* Batched form: ngraphs condensed graphs share one launch sequence. Graph g's lists are
+ * given as its own CSR (local ids); inside the library its ids are shifted by a 32-aligned
+ * base, and every task keeps its graph, so each graph's enumeration is the one it has
+ * alone. Tables are shared (one gamma pair and minimum size for the batch). */
+dqc_status dqc_upload_graph_batch(dqc_handle* h, int32_t ngraphs, const int32_t* nverts,
+                                  const uint64_t* const* out_off, const int32_t* const* out_nbr,
+                                  const uint64_t* const* in_off, const int32_t* const* in_nbr,
+                                  const uint64_t* const* two_off, const int32_t* const* two_nbr);
+/* Queues graph gid's level-0 task (records with local ids, as dqc_push_root). */
+dqc_status dqc_push_graph_root(dqc_handle* h, int32_t gid, const uint32_t* records, int32_t nvert, int32_t ncov);
+/* As dqc_fetch_cliques, with each clique's graph in graph[] and local ids. */
+dqc_status dqc_fetch_batch_cliques(dqc_handle* h, int64_t* count, int64_t* total_ids, int32_t* graph, int32_t* sizes,
+                                   int32_t* ids);
 
 /* Minimum-degree tables md[k] = ceil(gamma (k-1)) and bound tables ub[d] = (int)(d/gamma),
  * len entries each (reference src/host_functions.cu:9-15, :2165-2166). */
@@ -105,12 +131,42 @@ dqc_status dqc_run(dqc_handle* h, int32_t sched);
 dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t nm, int32_t ncov,
                          int32_t mode);
 
-/* Reads the task stack: meta gets 4 int32 per task {record offset, size, members, mode}
- * and records the 6-uint32 records. NULL buffers query the sizes. */
+/* Reads the task stack: meta gets 8 int32 per task {record offset, size, members, mode,
+ * id, parent id, depth, flags} and records the 6-uint32 records. NULL buffers query the
+ * sizes. */
 dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, int32_t* meta, uint32_t* records);
 
 /* Empties the task stack and the clique store (the graph and tables stay). */
 dqc_status dqc_reset(dqc_handle* h);
+
+/* Task log for the scheduling model (no counterpart in the reference): keeps up to
+ * `capacity` dqc_task_log records of the tasks expanded from now on (0 = off). */
+dqc_status dqc_set_task_log(dqc_handle* h, int64_t capacity);
+/* *count records are copied to `records` (NULL queries); *dropped = records beyond the
+ * capacity. */
+dqc_status dqc_fetch_task_log(dqc_handle* h, int64_t* count, int64_t* dropped, dqc_task_log* records);
+
+/* Subtree-cost model of the learned schedule (no counterpart in the reference): an MLP
+ * 7 -> 64 -> 64 -> 1 (ReLU, fp32) over the features of dqc_task_features, predicting
+ * log(1 + branches in the task's subtree). Parameters, DQC_MLP_PARAMS floats: W1[64][7],
+ * b1[64], W2[64][64], b2[64], W3[64], b3. Forward, gradient and training run as HIP
+ * kernels on the current device; all buffers are host memory. */
+#define DQC_MLP_IN 7
+#define DQC_MLP_HIDDEN 64
+#define DQC_MLP_PARAMS (64 * 7 + 64 + 64 * 64 + 64 + 64 + 1)
+/* x[7 * i ..] = {log1p(members), log1p(candidates), log1p(depth), log1p(min out-degree),
+ * log1p(max out-degree), log1p(min in-degree), log1p(max in-degree)} of record i. */
+dqc_status dqc_task_features(const dqc_task_log* records, int64_t n, float* x);
+dqc_status dqc_mlp_forward(const float* params, const float* x, int64_t n, float* y);
+/* Gradient of the mean squared error mean_i (y_i - t_i)^2 over all n samples. */
+dqc_status dqc_mlp_gradient(const float* params, const float* x, const float* t, int64_t n, float* grad,
+                            double* loss);
+/* Adam on minibatches of `batch` samples (shuffled by seed) for `epochs` passes; params
+ * is updated in place; *loss = final full-data mean squared error. */
+dqc_status dqc_mlp_train(float* params, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
+                         float lr, uint64_t seed, double* loss);
+/* Installs the model DQC_SCHED_LEARNED orders and splits batches by. */
+dqc_status dqc_set_cost_model(dqc_handle* h, const float* params, int64_t n_params);
 
 /* Runs at most `launches` expansion batches (one batch = the tasks popped from the top
  * of the task stack and expanded by one kernel launch); *finished = 1 once the stack is
@@ -158,6 +214,17 @@ dqc_status dqc_load_graph(const char* graph_path, int32_t with_twohop, int32_t* 
 dqc_status dqc_level0(const char* graph_path, double out_gamma, double in_gamma, int32_t min_size,
                       int32_t* n_condensed, int32_t* ncov, uint32_t** records, int32_t** orig_of);
 void dqc_free(void* p);
+
+/* Host preprocessing of a batch (each graph as dqc_prepare_file, one thread per graph)
+ * + upload of all graphs and their level-0 tasks into one handle; n_condensed[g] gets
+ * graph g's condensed size. */
+dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int32_t ngraphs, double out_gamma,
+                             double in_gamma, int32_t min_size, int32_t* n_condensed);
+/* The whole path for a batch of graphs in one handle: max_paths[g] (NULL entries skip)
+ * gets graph g's maximal file in RemoveNonMax format, max_cliques[g] its count. */
+dqc_status dqc_search_batch(const char* const* graph_paths, int32_t ngraphs, double out_gamma, double in_gamma,
+                            int32_t min_size, const char* const* max_paths, const dqc_config* cfg,
+                            int64_t* max_cliques, dqc_stats* st);
 
 /* RemoveNonMax (reference inc/Quick_rmnonmax.h:6): returns the number of maximal sets
  * written, 0 when the input holds none (no output file then), <0 on I/O error. */
