@@ -45,7 +45,7 @@ class Config(ctypes.Structure):
         ("clique_sets", ctypes.c_uint64), ("clique_ids", ctypes.c_uint64),
         ("host_addfail_check", ctypes.c_int32),
         ("split_branches", ctypes.c_int32), ("split_chunk", ctypes.c_int32),
-        ("schedule", ctypes.c_int32),
+        ("schedule", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
     ]
 
 

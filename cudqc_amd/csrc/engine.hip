@@ -62,7 +62,7 @@ constexpr unsigned long long SIZE_MASK = (1ull << CNT_SHIFT) - 1;
 constexpr unsigned long long MAX_ENTRIES = (1ull << 30) - 1;
 constexpr unsigned long long MAX_RECORDS = 1ull << 33;
 
-enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u };
+enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u, ERR_BATCH = 8u };
 
 // flags of a task descriptor
 constexpr int32_t TF_PIECE = 1;         // a piece of a split task: not counted, not split again
@@ -94,9 +94,12 @@ static_assert(sizeof(TaskLog) == 36, "task log layout");
 // Per-queue counters (one per task stack / stream).
 struct QCtr {
     unsigned long long next_task;  // next work item of the current launch
-    unsigned long long sp_alloc;   // task stack top: (tasks << 40) | records
+    unsigned long long sp_alloc;   // task stack top: (tasks << 34) | records
     unsigned long long root_first; // level-0 split: (branch << 32) | stack index of the first child
-    unsigned long long pad;
+    unsigned long long ntask;      // device-resident steps: tasks in the current batch
+    unsigned long long first;      //   its first stack index
+    unsigned long long v0;         //   and its first record
+    unsigned long long pad[2];
 };
 // Counters shared by every launch of a handle.
 struct GCtr {
@@ -107,6 +110,7 @@ struct GCtr {
     unsigned int pad;
     unsigned long long next_uid;   // task ids handed out (the level-0 task is id 0)
     unsigned long long log_n;      // task-log records requested
+    unsigned long long steps;      // device-resident steps that expanded a batch
 };
 
 struct KParams {
@@ -117,7 +121,7 @@ struct KParams {
     int32_t min_size, host_addfail, profile;
     const TaskDesc* in_desc;
     const dqc::TaskVertex* in_vtx;
-    unsigned long long in_ntask;
+    unsigned long long in_ntask;  // ~0ull: the batch size is q->ntask (device-resident steps)
     TaskDesc* sp_desc;
     dqc::TaskVertex* sp_vtx;
     unsigned long long sp_cap_t, sp_cap_v;
@@ -1198,7 +1202,7 @@ void expand_kernel(KParams P) {
         unsigned long long t = 0;
         if (threadIdx.x == 0) t = atomicAdd(&P.q->next_task, 1ull);
         t = w.bcast0(t);
-        if (t >= P.in_ntask) break;
+        if (t >= (P.in_ntask == ~0ull ? P.q->ntask : P.in_ntask)) break;
         if (P.split_s) {
             const TaskDesc d = P.in_desc[0];
             w.set_graph(d.gid);
@@ -1221,11 +1225,189 @@ void expand_kernel(KParams P) {
     }
 }
 
+// ---------------------------------------------------------------- device-resident steps
+// One expansion step without host round trips: pop_kernel chooses the batch (the same
+// rules as the host pop: at most pool_tasks newest tasks whose records fit the batch
+// buffer and whose worst-case children fit the stack), the optional ordering kernels
+// permute it, gather_kernel copies it into the batch buffer, expand_kernel runs it.
+
+struct PopParams {
+    TaskDesc* sdesc;
+    QCtr* q;
+    GCtr* g;
+    unsigned long long pool_tasks, pool_vertices, spill_tasks, spill_vertices;
+};
+
+constexpr int POP_T = 1024;
+
+// Single workgroup. Task i of the last m (i = 0 oldest) is admissible when every task
+// from i to the top fits: records[i..] within the batch buffer, worst-case output
+// within the space left after popping them. The batch starts at the smallest i such
+// that all of i..m-1 are admissible (the newest task always is).
+__global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
+    __shared__ unsigned long long csum_r[POP_T], csum_d[POP_T];
+    __shared__ int first_bad;
+    __shared__ unsigned long long carry_r, carry_d;
+    const QCtr qv = *pp.q;
+    const unsigned long long st_t = qv.sp_alloc >> CNT_SHIFT, st_v = qv.sp_alloc & SIZE_MASK;
+    const unsigned long long m = st_t < pp.pool_tasks ? st_t : pp.pool_tasks;
+    const int tid = threadIdx.x;
+    if (tid == 0) { first_bad = -1; carry_r = 0; carry_d = 0; }
+    __syncthreads();
+    // walk from the newest task down in chunks of POP_T, accumulating worst-case output
+    for (unsigned long long c0 = 0; c0 < m; c0 += POP_T) {
+        const unsigned long long j = c0 + tid;  // j-th newest
+        unsigned long long wr = 0, wd = 0;
+        TaskDesc d{};
+        if (j < m) {
+            d = pp.sdesc[st_t - 1 - j];
+            const unsigned long long steps = (unsigned long long)max(0, (d.flags & TF_PIECE) ? d.iters : d.tot - d.nm);
+            wr = steps * (unsigned long long)d.tot;
+            wd = steps + MAX_PIECES;
+        }
+        csum_r[tid] = wr;
+        csum_d[tid] = wd;
+        __syncthreads();
+        for (int off = 1; off < POP_T; off <<= 1) {  // inclusive scan
+            unsigned long long a = tid >= off ? csum_r[tid - off] : 0, b = tid >= off ? csum_d[tid - off] : 0;
+            __syncthreads();
+            csum_r[tid] += a;
+            csum_d[tid] += b;
+            __syncthreads();
+        }
+        if (j < m && j > 0) {
+            const unsigned long long recs = carry_r + csum_r[tid], descs = carry_d + csum_d[tid];
+            const unsigned long long idx = st_t - 1 - j;
+            const bool fits_batch = st_v - d.off <= pp.pool_vertices;
+            const bool fits_stack = recs <= pp.spill_vertices - d.off && descs <= pp.spill_tasks - idx;
+            if (!fits_batch || !fits_stack) atomicMin((unsigned*)&first_bad, (unsigned)j);  // -1 = none
+        }
+        __syncthreads();
+        if (tid == POP_T - 1) { carry_r += csum_r[tid]; carry_d += csum_d[tid]; }
+        __syncthreads();
+        if (first_bad >= 0) break;
+    }
+    if (tid == 0) {
+        unsigned long long cnt = first_bad >= 0 ? (unsigned long long)first_bad : m;
+        // nothing runs after an error, and a newest task larger than the batch buffer is one
+        if (pp.g->error) cnt = 0;
+        else if (m > 0 && st_v - pp.sdesc[st_t - 1].off > pp.pool_vertices) {
+            atomicOr(&pp.g->error, (unsigned)ERR_BATCH);
+            cnt = 0;
+        }
+        if (cnt > 0) atomicAdd(&pp.g->steps, 1ull);
+        QCtr& q = *pp.q;
+        q.next_task = 0;
+        q.ntask = cnt;
+        if (cnt > 0) {
+            const unsigned long long first = st_t - cnt;
+            const unsigned long long v0 = pp.sdesc[first].off;
+            q.first = first;
+            q.v0 = v0;
+            q.sp_alloc = (first << CNT_SHIFT) | v0;
+        }
+    }
+}
+
+// Batch order for the SIZE / LEARNED schedules, one workgroup: tasks go into 64 buckets
+// of descending log2 cost (cost: SIZE = steps x size, LEARNED = the model's prediction
+// in cost[], already computed), and in that order the eligible costliest tasks are
+// marked for splitting while their piece records fit `budget`. perm[k] = the batch index
+// of the k-th task to run; split[k] = 1 to split it.
+__global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
+                                                      const float* __restrict__ pred, int learned, int split_b,
+                                                      int split_c, unsigned long long budget, uint32_t* __restrict__ perm,
+                                                      uint8_t* __restrict__ split) {
+    __shared__ unsigned hist[64], start[64];
+    __shared__ unsigned long long scan[POP_T];
+    __shared__ unsigned long long carry;
+    const unsigned long long n = q->ntask, first = q->first;
+    const int tid = threadIdx.x;
+    if (tid < 64) hist[tid] = 0;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    auto bucket = [&](unsigned long long i) {
+        const TaskDesc d = sdesc[first + i];
+        const double steps = (double)max(1, d.tot - d.nm);
+        double c;
+        if (learned) {
+            const double share = (d.flags & TF_PIECE) ? fmin(1.0, (double)d.iters / steps) : 1.0;
+            c = expm1((double)fminf(pred[i], 60.f)) * share;
+        } else {
+            c = ((d.flags & TF_PIECE) ? (double)d.iters : steps) * (double)d.tot;
+        }
+        int b = (int)(2.0 * log2(1.0 + fmax(c, 0.0)));
+        return 63 - min(max(b, 0), 63);  // bucket 0 = costliest
+    };
+    for (unsigned long long i = tid; i < n; i += POP_T) atomicAdd(&hist[bucket(i)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        unsigned a = 0;
+        for (int b = 0; b < 64; ++b) { start[b] = a; a += hist[b]; }
+    }
+    __syncthreads();
+    for (unsigned long long i = tid; i < n; i += POP_T) perm[atomicAdd(&start[bucket(i)], 1u)] = (uint32_t)i;
+    __syncthreads();
+    // split marks in run order: eligible tasks while the running piece records fit
+    for (unsigned long long c0 = 0; c0 < n; c0 += POP_T) {
+        const unsigned long long k = c0 + tid;
+        unsigned long long need = 0;
+        if (k < n && split_b > 0) {
+            const TaskDesc d = sdesc[first + perm[k]];
+            const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
+            if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b) {
+                const int chunk = max(split_c, (steps + MAX_PIECES - 1) / MAX_PIECES);
+                need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
+            }
+        }
+        scan[tid] = need;
+        __syncthreads();
+        for (int off = 1; off < POP_T; off <<= 1) {
+            unsigned long long a = tid >= off ? scan[tid - off] : 0;
+            __syncthreads();
+            scan[tid] += a;
+            __syncthreads();
+        }
+        if (k < n) split[k] = need > 0 && carry + scan[tid] <= budget;
+        __syncthreads();
+        if (tid == POP_T - 1) carry += scan[tid];
+        __syncthreads();
+    }
+}
+
+// Copies the popped batch into the batch buffer: descriptors rebased (in perm order with
+// split marks when perm is given) and the records.
+__global__ void gather_kernel(const TaskDesc* __restrict__ sdesc, const dqc::TaskVertex* __restrict__ svtx,
+                              const QCtr* __restrict__ q, const uint32_t* __restrict__ perm,
+                              const uint8_t* __restrict__ split, TaskDesc* __restrict__ bdesc,
+                              dqc::TaskVertex* __restrict__ bvtx) {
+    const unsigned long long n = q->ntask;
+    if (n == 0) return;
+    const unsigned long long first = q->first, v0 = q->v0;
+    const TaskDesc last = sdesc[first + n - 1];
+    const unsigned long long vend = last.off + (unsigned long long)last.tot;
+    const unsigned long long gid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long gsz = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long k = gid; k < n; k += gsz) {
+        TaskDesc d = sdesc[first + (perm ? perm[k] : k)];
+        d.off -= v0;
+        if (perm && split[k]) d.flags |= TF_SPLIT;
+        bdesc[k] = d;
+    }
+    for (unsigned long long i = gid; i < vend - v0; i += gsz) bvtx[i] = svtx[v0 + i];
+}
+
 // Predicted log(1 + subtree branches) of tasks d[0..n) (the learned schedule's estimate);
 // the model is staged in LDS, one task per thread.
+// With q given, the tasks are the current device-resident batch on the stack.
 __global__ __launch_bounds__(256) void desc_cost_kernel(const float* __restrict__ params, const TaskDesc* __restrict__ d,
-                                                        unsigned long long n, float* __restrict__ y) {
+                                                        unsigned long long n, float* __restrict__ y,
+                                                        const QCtr* __restrict__ q) {
     __shared__ float p[dqc_cost::NPARAM];
+    if (q) {
+        n = q->ntask;
+        d += q->first;
+    }
     for (int i = threadIdx.x; i < dqc_cost::NPARAM; i += blockDim.x) p[i] = params[i];
     __syncthreads();
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1337,10 +1519,15 @@ struct dqc_handle {
     bool root_pending = false;
     int profile = 0;
     unsigned long long phase[8] = {0};
+    int64_t host_launches = 0;
     std::vector<TaskDesc> hbatch;  // scheduled batch descriptors (host copy for the upload)
     std::vector<double> hcost;
     TaskLog* d_log = nullptr;      // task log (dqc_set_task_log)
     unsigned long long log_cap = 0;
+    // device-resident steps: one captured HIP graph of resident_steps steps per schedule
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t* d_perm = nullptr;    // batch order (SIZE / LEARNED)
+    uint8_t* d_mark = nullptr;     // split marks in that order
     float* d_model = nullptr;      // cost model (dqc_set_cost_model)
     float* d_cost = nullptr;       // per-task predictions of the current batch
     std::vector<float> hpred;
@@ -1362,6 +1549,14 @@ bool dev_alloc(dqc_handle* h, T** p, size_t count, const char* what) {
     h->allocs.push_back(q);
     *p = (T*)q;
     return true;
+}
+
+void drop_graphs(dqc_handle* h) {
+    for (auto& e : h->gexec)
+        if (e) {
+            hipGraphExecDestroy(e);
+            e = nullptr;
+        }
 }
 
 void free_graph(dqc_handle* h) {
@@ -1461,6 +1656,7 @@ dqc_status dqc_close(dqc_handle* h) {
     if (!h) return DQC_E_ARG;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (void* p : h->allocs) hipFree(p);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
@@ -1479,6 +1675,7 @@ static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint
                               const uint64_t* in_off, const int32_t* in_nbr, const uint64_t* two_off,
                               const int32_t* two_nbr) {
     hipSetDevice(h->device);
+    drop_graphs(h);  // captured steps hold the old graph's pointers
     free_graph(h);
     h->n = n;
     h->kloc = std::max(kloc, 1);
@@ -1593,6 +1790,7 @@ dqc_status dqc_set_tables(dqc_handle* h, const int32_t* md_out, const int32_t* m
                           const int32_t* ub_in, int32_t len, int32_t min_size) {
     if (!h || len < 2 || min_size < 2 || min_size >= len) return DQC_E_ARG;
     hipSetDevice(h->device);
+    drop_graphs(h);
     if (h->d_mdo) {
         for (void* p : {(void*)h->d_mdo, (void*)h->d_mdi, (void*)h->d_ubo, (void*)h->d_ubi}) {
             hipFree(p);
@@ -1806,7 +2004,7 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
             return DQC_E_ARG;
         }
         const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-        desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc + first, n, h->d_cost);
+        desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc + first, n, h->d_cost, nullptr);
         h->hpred.resize(n);
         if (!hip_ok(h, hipGetLastError(), "cost launch") ||
             !hip_ok(h, hipMemcpyAsync(h->hpred.data(), h->d_cost, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream), "cost read") ||
@@ -1867,6 +2065,7 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
 }
 
 static void fold_stats(dqc_handle* h, const GCtr& g) {
+    h->st.launches = h->host_launches + (int64_t)g.steps;
     h->st.tasks_expanded = (int64_t)g.tasks + h->extra_tasks;
     h->st.branches = (int64_t)g.branches;
     h->st.raw_cliques = (int64_t)g.cliques + (int64_t)h->host_cliques.size();
@@ -1896,7 +2095,7 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     float ms = 0.f;
     hipEventElapsedTime(&ms, h->ev0, h->ev1);
     h->st.expand_kernel_ms += ms;
-    h->st.launches++;
+    h->host_launches++;
     fold_stats(h, g);
     if (g.error) {
         h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
@@ -1969,6 +2168,85 @@ static dqc_status split_root(dqc_handle* h) {
     return DQC_OK;
 }
 
+// Enqueues one device-resident step on the handle's stream: pop, (costs, order), gather,
+// expand. Nothing in it waits for the host.
+static void enqueue_step(dqc_handle* h, int32_t sched) {
+    PopParams pp{h->d_sdesc, h->d_q, h->d_g, h->cfg.pool_tasks, h->cfg.pool_vertices, h->cfg.spill_tasks,
+                 h->cfg.spill_vertices};
+    pop_kernel<<<1, POP_T, 0, h->stream>>>(pp);
+    const uint32_t* perm = nullptr;
+    const uint8_t* mark = nullptr;
+    if (sched != DQC_SCHED_FIFO) {
+        if (sched == DQC_SCHED_LEARNED) {
+            const int blocks = (int)std::min<unsigned long long>(1024, (h->cfg.pool_tasks + 255) / 256);
+            desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc, 0, h->d_cost, h->d_q);
+        }
+        order_kernel<<<1, POP_T, 0, h->stream>>>(h->d_sdesc, h->d_q, h->d_cost, sched == DQC_SCHED_LEARNED,
+                                                 h->cfg.split_branches, h->cfg.split_chunk, h->cfg.pool_vertices / 16,
+                                                 h->d_perm, h->d_mark);
+        perm = h->d_perm;
+        mark = h->d_mark;
+    }
+    gather_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, h->d_q, perm, mark, h->d_bdesc, h->d_bvtx);
+    const KParams P = make_params(h, ~0ull);
+    if (h->big) expand_kernel<true><<<h->launch_wg, NT, 0, h->stream>>>(P);
+    else expand_kernel<false><<<h->launch_wg, NT, 0, h->stream>>>(P);
+}
+
+// Device-resident expansion: replays a captured HIP graph of resident_steps steps until
+// the stack is empty (or max_steps steps ran); the host only reads the counters between
+// replays.
+static dqc_status resident_run(dqc_handle* h, int32_t sched, int64_t max_steps, int64_t* done) {
+    if (sched == DQC_SCHED_LEARNED && !h->d_model) { h->err = "no cost model set (dqc_set_cost_model)"; return DQC_E_ARG; }
+    if (!h->d_perm && (!dev_alloc(h, &h->d_perm, h->cfg.pool_tasks, "order alloc") ||
+                       !dev_alloc(h, &h->d_mark, h->cfg.pool_tasks, "order alloc")))
+        return DQC_E_HIP;
+    QCtr q{};
+    q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
+    q.root_first = ~0ull;
+    if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
+    hipGraphExec_t& ex = h->gexec[sched];
+    const int S = h->cfg.resident_steps;
+    if (!ex) {
+        hipGraph_t g = nullptr;
+        if (!hip_ok(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed), "capture")) return DQC_E_HIP;
+        for (int i = 0; i < S; ++i) enqueue_step(h, sched);
+        if (!hip_ok(h, hipStreamEndCapture(h->stream, &g), "capture") ||
+            !hip_ok(h, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "graph instantiate")) {
+            if (g) hipGraphDestroy(g);
+            ex = nullptr;
+            return DQC_E_HIP;
+        }
+        hipGraphDestroy(g);
+    }
+    while (max_steps < 0 || *done < max_steps) {
+        hipEventRecord(h->ev0, h->stream);
+        if (!hip_ok(h, hipGraphLaunch(ex, h->stream), "graph launch")) return DQC_E_HIP;
+        hipEventRecord(h->ev1, h->stream);
+        GCtr g{};
+        if (!hip_ok(h, hipMemcpyAsync(&q, h->d_q, sizeof q, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
+            !hip_ok(h, hipMemcpyAsync(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost, h->stream), "ctr read") ||
+            !hip_ok(h, hipStreamSynchronize(h->stream), "expand graph"))
+            return DQC_E_HIP;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        h->st.expand_kernel_ms += ms;
+        *done += S;
+        fold_stats(h, g);
+        h->st_t = q.sp_alloc >> CNT_SHIFT;
+        h->st_v = q.sp_alloc & SIZE_MASK;
+        if (g.error) {
+            h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
+                   : (g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
+                   : (g.error & ERR_BATCH) ? "a task does not fit the batch buffer (raise pool_vertices)"
+                   : "a task has more members than the member histograms hold";
+            return (g.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
+        }
+        if (h->st_t == 0) break;
+    }
+    return DQC_OK;
+}
+
 static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launches, int32_t* finished) {
     if (!h || sched < DQC_SCHED_DEFAULT || sched > DQC_SCHED_LEARNED) return DQC_E_ARG;
     if (sched == DQC_SCHED_DEFAULT) sched = h->cfg.schedule;
@@ -1983,6 +2261,10 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
         if (h->st_t == 1 && h->root_pending) {
             h->root_pending = false;
             dqc_status rc = split_root(h);
+            if (rc != DQC_OK) return rc;
+        } else if (h->cfg.resident_steps > 0) {
+            --done;
+            dqc_status rc = resident_run(h, sched, max_launches, &done);
             if (rc != DQC_OK) return rc;
         } else {
             unsigned long long m = 0;
@@ -2028,6 +2310,7 @@ dqc_status dqc_set_cost_model(dqc_handle* h, const float* params, int64_t n_para
 dqc_status dqc_set_task_log(dqc_handle* h, int64_t capacity) {
     if (!h || capacity < 0) return DQC_E_ARG;
     hipSetDevice(h->device);
+    drop_graphs(h);
     if (h->d_log) {
         hipFree(h->d_log);
         h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->d_log), h->allocs.end());

@@ -46,6 +46,10 @@ typedef struct {
     int32_t split_chunk;       /* least branches per piece; 0 = 32                    */
     int32_t schedule;          /* DQC_SCHED_* used by dqc_search_file and by
                                   DQC_SCHED_DEFAULT; 0 = DQC_SCHED_SIZE               */
+    int32_t resident_steps;    /* > 0: batches are popped, ordered and expanded on the
+                                  device, resident_steps per replay of one captured HIP
+                                  graph, with no host round trip between them;
+                                  0 = host-driven steps                               */
 } dqc_config;
 
 typedef struct {

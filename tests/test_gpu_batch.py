@@ -51,3 +51,10 @@ def test_batch_fifo_and_split(dqc, manifest, tmp_path):
 def test_batch_hbm_bitmaps(dqc, manifest, tmp_path, monkeypatch):
     monkeypatch.setenv("DQC_HBM_BITMAPS", "1")
     check(dqc, batch_cases(manifest), tmp_path)
+
+
+def test_batch_resident(dqc, manifest, tmp_path):
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.resident_steps = 8
+    check(dqc, batch_cases(manifest), tmp_path, config=cfg)

@@ -1,0 +1,95 @@
+"""Schedules and device-resident steps: the output and the tree counts never depend on
+the order tasks run in, whether batches are chosen by the host or on the device (one
+captured HIP graph per sequence of steps), or whether a trained cost model orders them.
+Also measures what the learned order gains over plain FIFO on a complete enumeration
+(the 256-graph batch workload)."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def run_golden(dqc, manifest, tmp_path, cfg, model=None):
+    bad = []
+    for c in manifest["cases"]:
+        path = os.path.join(GOLDEN, c["graph"])
+        with dqc.Engine(cfg) as eng:
+            eng.prepare(path, c["out_gamma"], c["in_gamma"], c["min_size"])
+            if model is not None:
+                eng.set_cost_model(model)
+            eng.run()
+            st = eng.stats()
+            cl = eng.cliques()
+        mx = str(tmp_path / (c["name"] + ".max"))
+        if os.path.exists(mx):
+            os.remove(mx)
+        raw = str(tmp_path / (c["name"] + ".raw"))
+        with open(raw, "w") as f:
+            for s in cl:
+                f.write(f"{len(s)} " + "".join(f"{x} " for x in s) + "\n")
+        cnt = dqc.remove_non_max(raw, mx) if cl else 0
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
+                or st.raw_cliques != c["raw_cliques"]):
+            bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"], st.branches, c["branches"]))
+    return bad
+
+
+def config(dqc, schedule, resident=0, split=(0, 0)):
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.schedule = schedule
+    cfg.resident_steps = resident
+    cfg.split_branches, cfg.split_chunk = split
+    return cfg
+
+
+def test_resident_fifo(dqc, manifest, tmp_path):
+    assert not run_golden(dqc, manifest, tmp_path, config(dqc, dqc.SCHED_FIFO, resident=4))
+
+
+def test_resident_size_split_everything(dqc, manifest, tmp_path):
+    assert not run_golden(dqc, manifest, tmp_path, config(dqc, dqc.SCHED_SIZE, resident=3, split=(2, 3)))
+
+
+def test_learned_host_and_resident(dqc, manifest, tmp_path):
+    from cudqc_amd import learn
+    model = learn.load_model()
+    assert not run_golden(dqc, manifest, tmp_path, config(dqc, dqc.SCHED_LEARNED, split=(2, 3)), model)
+    assert not run_golden(dqc, manifest, tmp_path, config(dqc, dqc.SCHED_LEARNED, resident=4), model)
+
+
+def test_learned_gain_on_batch_workload(dqc, tmp_path):
+    """The 256-graph batch, enumerated completely, plain FIFO vs the learned schedule
+    (both device-resident). Identical counts; the timings are printed."""
+    from cudqc_amd import learn
+    paths = []
+    for g in range(256):
+        p = str(tmp_path / f"w3_{g}.txt")
+        dqc.generate(3, p, g)
+        paths.append(p)
+    out = {}
+    for name, sched in (("fifo", dqc.SCHED_FIFO), ("learned", dqc.SCHED_LEARNED), ("fifo2", dqc.SCHED_FIFO),
+                        ("learned2", dqc.SCHED_LEARNED)):
+        cfg = config(dqc, sched, resident=16)
+        with dqc.Engine(cfg) as eng:
+            eng.prepare_batch(paths, 0.85, 0.85, 10)
+            if sched == dqc.SCHED_LEARNED:
+                eng.set_cost_model(learn.load_model())
+            t = time.perf_counter()
+            eng.run()
+            wall = time.perf_counter() - t
+            st = eng.stats()
+            out[name] = (wall, st.expand_kernel_ms, st.tasks_expanded, st.branches, st.raw_cliques)
+        print(f"{name}: wall {wall:.3f}s kernel {st.expand_kernel_ms:.0f}ms tasks {st.tasks_expanded} "
+              f"branches {st.branches} raw {st.raw_cliques}", flush=True)
+    assert len({v[2:] for v in out.values()}) == 1
+    fifo = min(out["fifo"][0], out["fifo2"][0])
+    learned = min(out["learned"][0], out["learned2"][0])
+    print(f"learned gain over FIFO (wall): {fifo / learned:.3f}x", flush=True)
