@@ -45,7 +45,7 @@ class Config(ctypes.Structure):
         ("clique_sets", ctypes.c_uint64), ("clique_ids", ctypes.c_uint64),
         ("host_addfail_check", ctypes.c_int32),
         ("split_branches", ctypes.c_int32), ("split_chunk", ctypes.c_int32),
-        ("schedule", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
+        ("schedule", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
     ]
 
 
@@ -58,6 +58,8 @@ class Stats(ctypes.Structure):
         ("maximal_cliques", ctypes.c_int64), ("bytes_scanned", ctypes.c_int64), ("launches", ctypes.c_int64),
         ("slow_sorts", ctypes.c_int64), ("expand_kernel_ms", ctypes.c_double), ("expand_wall_ms", ctypes.c_double),
         ("n_condensed", ctypes.c_int32), ("workgroups", ctypes.c_int32),
+        ("truncated_children", ctypes.c_int64), ("stack_pushbacks", ctypes.c_int64),
+        ("peak_stack_tasks", ctypes.c_int64), ("peak_stack_records", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -112,6 +114,8 @@ def load():
     lib.dqc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.dqc_get_phases.restype = ctypes.c_int32
     lib.dqc_get_phases.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    lib.dqc_push_root.restype = ctypes.c_int32
+    lib.dqc_push_root.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int32, ctypes.c_int32]
     lib.dqc_push_task.restype = ctypes.c_int32
     lib.dqc_push_task.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.dqc_fetch_tasks.restype = ctypes.c_int32
@@ -309,6 +313,11 @@ class Engine:
     def reset(self):
         self._check(self.lib.dqc_reset(self.h), "dqc_reset")
 
+    def push_root(self, records, ncov):
+        """Queues a level-0 task (expanded branch-parallel on a single graph)."""
+        flat = (ctypes.c_uint32 * (6 * len(records)))(*[x for r in records for x in r])
+        self._check(self.lib.dqc_push_root(self.h, flat, len(records), ncov), "dqc_push_root")
+
     def push_task(self, records, nm, ncov=0, mode=2):
         """records: list of 6-tuples (id, out_mem, out_can, in_mem, in_can, lvl2adj)."""
         flat = (ctypes.c_uint32 * (6 * len(records)))(*[x for r in records for x in r])
@@ -318,12 +327,12 @@ class Engine:
         """The task stack as a list of (records, nm, mode)."""
         nt, nr = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
-        meta = (ctypes.c_int32 * max(8 * nt.value, 1))()
+        meta = (ctypes.c_int32 * max(9 * nt.value, 1))()
         rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
         out = []
         for i in range(nt.value):
-            off, tot, nm, mode = meta[8 * i:8 * i + 4]
+            off, tot, nm, mode = meta[9 * i:9 * i + 4]
             recs = [tuple(rec[6 * (off + j):6 * (off + j) + 6]) for j in range(tot)]
             out.append((recs, nm, mode))
         return out
@@ -365,10 +374,23 @@ class Engine:
         """(uid, parent) of every task on the stack."""
         nt, nr = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
-        meta = (ctypes.c_int32 * max(8 * nt.value, 1))()
+        meta = (ctypes.c_int32 * max(9 * nt.value, 1))()
         rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
-        return [(meta[8 * i + 4] & 0xFFFFFFFF, meta[8 * i + 5] & 0xFFFFFFFF) for i in range(nt.value)]
+        return [(meta[9 * i + 4] & 0xFFFFFFFF, meta[9 * i + 5] & 0xFFFFFFFF) for i in range(nt.value)]
+
+    def task_meta(self):
+        """(records, nm, mode, ncov) of every task on the stack."""
+        nt, nr = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
+        meta = (ctypes.c_int32 * max(9 * nt.value, 1))()
+        rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
+        out = []
+        for i in range(nt.value):
+            off, tot, nm, mode = meta[9 * i:9 * i + 4]
+            out.append(([tuple(rec[6 * (off + j):6 * (off + j) + 6]) for j in range(tot)], nm, mode, meta[9 * i + 8]))
+        return out
 
     def set_cost_model(self, params):
         import numpy as np

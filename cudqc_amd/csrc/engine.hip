@@ -99,6 +99,11 @@ struct QCtr {
     unsigned long long ntask;      // device-resident steps: tasks in the current batch
     unsigned long long first;      //   its first stack index
     unsigned long long v0;         //   and its first record
+    unsigned long long maxtot;     //   and its largest task (the stack reserve is sized by it)
+    unsigned long long pend;       // batch tasks not yet finished: (tasks << 34) | records
+    unsigned long long wg_limit;   // workgroups the stack has room for (device-resident steps)
+    unsigned long long def_alloc;  // deferred region, growing down from the top of the stack
+                                   // arrays: (pieces << 34) | records
     unsigned long long pad[2];
 };
 // Counters shared by every launch of a handle.
@@ -111,6 +116,9 @@ struct GCtr {
     unsigned long long next_uid;   // task ids handed out (the level-0 task is id 0)
     unsigned long long log_n;      // task-log records requested
     unsigned long long steps;      // device-resident steps that expanded a batch
+    unsigned long long truncated;  // children not pushed: deeper than max_depth
+    unsigned long long spill_state[4];  // first failed stack write: top, request, launch state, pending
+    unsigned long long pushbacks;  // tasks whose rest went back to the stack for lack of room
 };
 
 struct KParams {
@@ -140,6 +148,8 @@ struct KParams {
     TaskLog* log;               // task log (null = off)
     unsigned long long log_cap;
     int32_t kloc;               // largest vertex count of one graph (scratch arrays, bitmaps)
+    uint32_t max_depth;         // children deeper than this are counted, not pushed (0 = no bound)
+    unsigned long long maxtot;  // largest task of the batch (~0ull: q->maxtot, device-resident steps)
     const int32_t* gbase;       // batched graphs: first id of each graph (multiples of 32); null = one graph
 };
 
@@ -204,6 +214,7 @@ struct W {
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
     uint32_t nbranch;             // branches run for the current task / piece
+    uint32_t launch_steps;        // stack checks this workgroup passed in this launch
     uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
     unsigned long long* hsort;  // HBM sort keys for children above SORT_N (2k entries)
@@ -623,6 +634,10 @@ struct W {
     // and pushes the child onto the task stack.
     __device__ __forceinline__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
         const int N = cm + nr;
+        if (P->max_depth && cur_depth + 1u > P->max_depth) {  // depth-bounded enumeration
+            if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
+            return -1;
+        }
         // key field widths from this child's maxima
         uint32_t mmx = 0, mcx = 0, l2x = 0;
         uint32_t omin = UINT_MAX, omax = 0, imin = UINT_MAX, imax = 0;  // degree features
@@ -658,8 +673,10 @@ struct W {
         if (tid == 0) {
             const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)N;
             a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(&P->q->sp_alloc, inc);
-            if (a != ~0ull && ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + N > P->sp_cap_v)) {
-                atomicOr(&P->g->error, (unsigned)ERR_SPILL);
+            const unsigned long long dq = __atomic_load_n(&P->q->def_alloc, __ATOMIC_RELAXED);
+            if (a != ~0ull && ((a >> CNT_SHIFT) + (dq >> CNT_SHIFT) >= P->sp_cap_t ||
+                               (a & SIZE_MASK) + (dq & SIZE_MASK) + N > P->sp_cap_v)) {
+                note_spill(a, (unsigned long long)N);
                 a = ~0ull;
             }
         }
@@ -746,21 +763,68 @@ struct W {
         return (long long)t;
     }
 
+    // Flags a stack overflow; the first one records the stack top, the request (bit 63:
+    // a piece), this workgroup's launch state and the pending batch share.
+    __device__ __forceinline__ void note_spill(unsigned long long top, unsigned long long req) {
+        if (atomicOr(&P->g->error, (unsigned)ERR_SPILL) & ERR_SPILL) return;
+        P->g->spill_state[0] = top;
+        P->g->spill_state[1] = req;
+        P->g->spill_state[2] = gridDim.x;
+        P->g->spill_state[3] = __atomic_load_n(&P->q->pend, __ATOMIC_RELAXED);
+    }
+
+    // Stack backpressure. After a step passes this check a workgroup writes at most one
+    // child (no larger than the batch's largest task) before its next check, and every
+    // batch task not yet finished may later be pushed back once as a piece of at most its
+    // own size. The stack stays within capacity as long as each check sees room for one
+    // child per resident workgroup plus all unfinished batch tasks; a workgroup that does
+    // not see it pushes the rest of its task as a piece and stops it. At launch start the
+    // pop leaves that room, so a workgroup's first check in a launch passes unless other
+    // workgroups already wrote children (no launch can stall). Its later checks also keep
+    // one task of the largest size (kloc) free, so that a launch leaves room for the next.
+    __device__ __forceinline__ bool stack_low() {
+        unsigned long long low = 0;
+        if (tid == 0) {
+            const unsigned long long a = __atomic_load_n(&P->q->sp_alloc, __ATOMIC_RELAXED) +
+                                         __atomic_load_n(&P->q->def_alloc, __ATOMIC_RELAXED);
+            const unsigned long long pend = __atomic_load_n(&P->q->pend, __ATOMIC_RELAXED);
+            const unsigned long long mt = P->maxtot == ~0ull ? P->q->maxtot : P->maxtot;
+            const unsigned long long wg = P->maxtot == ~0ull ? P->q->wg_limit : gridDim.x;
+            const unsigned long long used_t = a >> CNT_SHIFT, used_v = a & SIZE_MASK;
+            const unsigned long long margin = launch_steps ? (unsigned long long)P->kloc : 0ull;
+            low = used_t + wg + (pend >> CNT_SHIFT) + (launch_steps ? 1 : 0) > P->sp_cap_t ||
+                  used_v + wg * mt + (pend & SIZE_MASK) + margin > P->sp_cap_v;
+        }
+        const bool r = bcast0(low) != 0;
+        if (!r) ++launch_steps;
+        return r;
+    }
+
     // Pushes the current state of the task (live vertices in position order with their
     // current degrees and lvl2adj) as a piece that runs `iters` loop steps from here.
-    __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters) {
+    // deferred: into the deferred region (backpressure), which is expanded only when the
+    // main stack is empty, after the children written below the task have run.
+    __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters,
+                                                bool deferred = false) {
         unsigned long long a = 0;
         if (tid == 0) {
             const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)tot;
-            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(&P->q->sp_alloc, inc);
-            if (a != ~0ull && ((a >> CNT_SHIFT) >= P->sp_cap_t || (a & SIZE_MASK) + tot > P->sp_cap_v)) {
-                atomicOr(&P->g->error, (unsigned)ERR_SPILL);
+            unsigned long long* ctr = deferred ? &P->q->def_alloc : &P->q->sp_alloc;
+            unsigned long long* other = deferred ? &P->q->sp_alloc : &P->q->def_alloc;
+            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(ctr, inc);
+            const unsigned long long b = __atomic_load_n(other, __ATOMIC_RELAXED);
+            if (a != ~0ull && ((a >> CNT_SHIFT) + (b >> CNT_SHIFT) >= P->sp_cap_t ||
+                               (a & SIZE_MASK) + (b & SIZE_MASK) + tot > P->sp_cap_v)) {
+                note_spill(a, (unsigned long long)tot | (1ull << 63));
                 a = ~0ull;
             }
         }
         a = bcast0(a);
         if (a == ~0ull) return;
-        const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
+        // the deferred region fills the arrays from the top down: its newest piece has the
+        // lowest descriptor index and the lowest records, so its newest pieces stay contiguous
+        const unsigned long long t = deferred ? P->sp_cap_t - 1 - (a >> CNT_SHIFT) : a >> CNT_SHIFT;
+        const unsigned long long o = deferred ? P->sp_cap_v - (a & SIZE_MASK) - (unsigned long long)tot : a & SIZE_MASK;
         dqc::TaskVertex* dv = P->sp_vtx + o;
         for (int i = tid; i < tot; i += NT) {
             const int x = pids[i];
@@ -1120,6 +1184,11 @@ struct W {
         const bool remove_first = piece && (d.flags & TF_REMOVE_FIRST);
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
+            if (stack_low()) {  // backpressure: the rest of the task becomes a deferred piece
+                write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, true);
+                if (tid == 0) atomicAdd(&P->g->pushbacks, 1ull);
+                break;
+            }
             if (split && (j - ncov) % chunk == 0)
                 write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
             if (j != ncov || remove_first) {
@@ -1152,6 +1221,7 @@ struct W {
                 P->log[li] = r;
             }
         }
+        if (tid == 0) atomicSub(&P->q->pend, (1ull << CNT_SHIFT) | (unsigned long long)T);
         for (int i = tid; i < T; i += NT) {
             const int wd = (pids[i] - gb) >> 5;
             plive[wd] = 0; pmem[wd] = 0;
@@ -1167,6 +1237,7 @@ struct W {
 template <bool BIG>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DQC_WAVES_PER_EU)))
 void expand_kernel(KParams P) {
+    if (P.in_ntask == ~0ull && blockIdx.x >= P.q->wg_limit) return;  // no stack room for them
     __shared__ Shared S;
     W w;
     w.P = &P;
@@ -1196,6 +1267,7 @@ void expand_kernel(KParams P) {
     w.plive = bm; w.pmem = bm + bw; w.cmem = bm + 2 * bw; w.ccand = bm + 3 * bw; w.cL = bm + 4 * bw; w.ccrit = bm + 5 * bw;
     w.dirty_m = 0;
     w.dirty_l = 0;
+    w.launch_steps = 0;
     if (threadIdx.x == 0) S.acc_bytes = 0;
     __syncthreads();
     while (true) {
@@ -1236,75 +1308,91 @@ struct PopParams {
     QCtr* q;
     GCtr* g;
     unsigned long long pool_tasks, pool_vertices, spill_tasks, spill_vertices;
+    unsigned long long workgroups;
 };
 
 constexpr int POP_T = 1024;
 
-// Single workgroup. Task i of the last m (i = 0 oldest) is admissible when every task
-// from i to the top fits: records[i..] within the batch buffer, worst-case output
-// within the space left after popping them. The batch starts at the smallest i such
-// that all of i..m-1 are admissible (the newest task always is).
+// Single workgroup. The batch is the newest tasks of the main stack, or of the deferred
+// region when the main stack is empty: at most pool_tasks, as many as fit the batch
+// buffer (the newest one always does, or the run stops with an error). The launch gets
+// as many workgroups as the stack has room for one child each; with no room for the
+// whole batch, its newest task alone is tried.
 __global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
-    __shared__ unsigned long long csum_r[POP_T], csum_d[POP_T];
     __shared__ int first_bad;
-    __shared__ unsigned long long carry_r, carry_d;
+    __shared__ unsigned maxtot;
     const QCtr qv = *pp.q;
     const unsigned long long st_t = qv.sp_alloc >> CNT_SHIFT, st_v = qv.sp_alloc & SIZE_MASK;
-    const unsigned long long m = st_t < pp.pool_tasks ? st_t : pp.pool_tasks;
+    const unsigned long long df_t = qv.def_alloc >> CNT_SHIFT, df_v = qv.def_alloc & SIZE_MASK;
+    const bool from_main = st_t > 0;
+    const unsigned long long avail = from_main ? st_t : df_t;
+    const unsigned long long m = avail < pp.pool_tasks ? avail : pp.pool_tasks;
+    // the j-th newest task, and the records from the newest task through it
+    auto idx = [&](unsigned long long j) { return from_main ? st_t - 1 - j : pp.spill_tasks - df_t + j; };
+    auto span = [&](const TaskDesc& d) {
+        return from_main ? st_v - d.off : d.off + (unsigned long long)d.tot - (pp.spill_vertices - df_v);
+    };
     const int tid = threadIdx.x;
-    if (tid == 0) { first_bad = -1; carry_r = 0; carry_d = 0; }
+    if (tid == 0) { first_bad = -1; maxtot = 0; }
     __syncthreads();
-    // walk from the newest task down in chunks of POP_T, accumulating worst-case output
     for (unsigned long long c0 = 0; c0 < m; c0 += POP_T) {
-        const unsigned long long j = c0 + tid;  // j-th newest
-        unsigned long long wr = 0, wd = 0;
-        TaskDesc d{};
-        if (j < m) {
-            d = pp.sdesc[st_t - 1 - j];
-            const unsigned long long steps = (unsigned long long)max(0, (d.flags & TF_PIECE) ? d.iters : d.tot - d.nm);
-            wr = steps * (unsigned long long)d.tot;
-            wd = steps + MAX_PIECES;
-        }
-        csum_r[tid] = wr;
-        csum_d[tid] = wd;
+        const unsigned long long j = c0 + tid;
+        if (j < m && j > 0 && span(pp.sdesc[idx(j)]) > pp.pool_vertices)  // records must fit the batch buffer
+            atomicMin((unsigned*)&first_bad, (unsigned)j);  // -1 = none
         __syncthreads();
-        for (int off = 1; off < POP_T; off <<= 1) {  // inclusive scan
-            unsigned long long a = tid >= off ? csum_r[tid - off] : 0, b = tid >= off ? csum_d[tid - off] : 0;
-            __syncthreads();
-            csum_r[tid] += a;
-            csum_d[tid] += b;
-            __syncthreads();
-        }
-        if (j < m && j > 0) {
-            const unsigned long long recs = carry_r + csum_r[tid], descs = carry_d + csum_d[tid];
-            const unsigned long long idx = st_t - 1 - j;
-            const bool fits_batch = st_v - d.off <= pp.pool_vertices;
-            const bool fits_stack = recs <= pp.spill_vertices - d.off && descs <= pp.spill_tasks - idx;
-            if (!fits_batch || !fits_stack) atomicMin((unsigned*)&first_bad, (unsigned)j);  // -1 = none
-        }
+        const bool stop = first_bad >= 0;  // read by every thread before any next update
         __syncthreads();
-        if (tid == POP_T - 1) { carry_r += csum_r[tid]; carry_d += csum_d[tid]; }
+        if (stop) break;
+    }
+    {
+        const unsigned long long cnt = first_bad >= 0 ? (unsigned long long)first_bad : m;
+        for (unsigned long long j = tid; j < cnt; j += POP_T) atomicMax(&maxtot, (unsigned)pp.sdesc[idx(j)].tot);
         __syncthreads();
-        if (first_bad >= 0) break;
     }
     if (tid == 0) {
         unsigned long long cnt = first_bad >= 0 ? (unsigned long long)first_bad : m;
         // nothing runs after an error, and a newest task larger than the batch buffer is one
         if (pp.g->error) cnt = 0;
-        else if (m > 0 && st_v - pp.sdesc[st_t - 1].off > pp.pool_vertices) {
+        else if (m > 0 && span(pp.sdesc[idx(0)]) > pp.pool_vertices) {
             atomicOr(&pp.g->error, (unsigned)ERR_BATCH);
             cnt = 0;
+        }
+        unsigned long long wgl = pp.workgroups, mtot = maxtot;
+        if (cnt > 0) {
+            const unsigned long long room_t = pp.spill_tasks - st_t - df_t, free_v = pp.spill_vertices - st_v - df_v;
+            wgl = min(wgl, min(free_v / (mtot ? mtot : 1), room_t));
+            if (wgl == 0) {  // no room for the whole batch: the newest task alone, if it fits
+                cnt = 1;
+                mtot = (unsigned long long)pp.sdesc[idx(0)].tot;
+                wgl = min(pp.workgroups, min(free_v / (mtot ? mtot : 1), room_t));
+            }
+            if (wgl == 0) {
+                atomicOr(&pp.g->error, (unsigned)ERR_SPILL);
+                cnt = 0;
+            }
+            // under stack pressure, two tasks per workgroup (a full batch would mostly be
+            // deferred untouched)
+            if (wgl < pp.workgroups && cnt > 2 * wgl) cnt = 2 * wgl;
         }
         if (cnt > 0) atomicAdd(&pp.g->steps, 1ull);
         QCtr& q = *pp.q;
         q.next_task = 0;
         q.ntask = cnt;
+        q.maxtot = mtot;
+        q.wg_limit = wgl;
+        q.pend = 0;
         if (cnt > 0) {
-            const unsigned long long first = st_t - cnt;
+            // first / v0: the batch's lowest descriptor index and record; both regions keep a
+            // batch contiguous (gather_kernel copies [v0, last.off + last.tot))
+            const unsigned long long first = from_main ? st_t - cnt : pp.spill_tasks - df_t;
             const unsigned long long v0 = pp.sdesc[first].off;
+            const TaskDesc last = pp.sdesc[first + cnt - 1];
+            const unsigned long long recs = last.off + (unsigned long long)last.tot - v0;
             q.first = first;
             q.v0 = v0;
-            q.sp_alloc = (first << CNT_SHIFT) | v0;
+            if (from_main) q.sp_alloc = (first << CNT_SHIFT) | v0;
+            else q.def_alloc = ((df_t - cnt) << CNT_SHIFT) | (df_v - recs);
+            q.pend = (cnt << CNT_SHIFT) | recs;
         }
     }
 }
@@ -1330,9 +1418,9 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
         const TaskDesc d = sdesc[first + i];
         const double steps = (double)max(1, d.tot - d.nm);
         double c;
-        if (learned) {
+        if (learned) {  // predicted branches x task size
             const double share = (d.flags & TF_PIECE) ? fmin(1.0, (double)d.iters / steps) : 1.0;
-            c = expm1((double)fminf(pred[i], 60.f)) * share;
+            c = expm1((double)fminf(pred[i], 60.f)) * share * (double)d.tot;
         } else {
             c = ((d.flags & TF_PIECE) ? (double)d.iters : steps) * (double)d.tot;
         }
@@ -1498,6 +1586,8 @@ struct dqc_handle {
     TaskDesc* d_sdesc = nullptr;
     dqc::TaskVertex* d_svtx = nullptr;
     unsigned long long st_t = 0, st_v = 0;  // stack depth: tasks, records
+    unsigned long long def_t = 0, def_v = 0;  // deferred region (from the top of the arrays down)
+    unsigned long long peak_t = 0, peak_v = 0;  // largest stack use seen between launches
     // cliques
     int32_t* d_clq_ids = nullptr;
     unsigned long long* d_clq_meta = nullptr;
@@ -1520,6 +1610,9 @@ struct dqc_handle {
     int profile = 0;
     unsigned long long phase[8] = {0};
     int64_t host_launches = 0;
+    unsigned long long batch_maxtot = 0;  // largest task of the host-popped batch
+    unsigned long long batch_pend = 0;    // its (tasks << 34) | records
+    unsigned long long batch_grid = 0;    // workgroups the stack has room for
     std::vector<TaskDesc> hbatch;  // scheduled batch descriptors (host copy for the upload)
     std::vector<double> hcost;
     TaskLog* d_log = nullptr;      // task log (dqc_set_task_log)
@@ -1836,8 +1929,8 @@ static dqc_status push_root_impl(dqc_handle* h, int32_t gid, const uint32_t* rec
     }
     for (int32_t i = 0; i < nvert; ++i)
         if (records[6 * (size_t)i] < (uint32_t)b || records[6 * (size_t)i] >= (uint32_t)(b + nloc)) return DQC_E_ARG;
-    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + 1 > h->cfg.spill_tasks ||
-        h->st_v + nvert > h->cfg.spill_vertices) {
+    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + h->def_t + 1 > h->cfg.spill_tasks ||
+        h->st_v + h->def_v + nvert > h->cfg.spill_vertices) {
         h->err = "level-0 task exceeds the batch or stack capacity";
         return DQC_E_CAPACITY;
     }
@@ -1862,8 +1955,8 @@ dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, 
     if (nvert > h->n) return DQC_E_ARG;
     for (int32_t i = 0; i < nvert; ++i)
         if (records[6 * i] >= (uint32_t)h->n) return DQC_E_ARG;
-    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + 1 > h->cfg.spill_tasks ||
-        h->st_v + nvert > h->cfg.spill_vertices) {
+    if ((unsigned long long)nvert > h->cfg.pool_vertices || h->st_t + h->def_t + 1 > h->cfg.spill_tasks ||
+        h->st_v + h->def_v + nvert > h->cfg.spill_vertices) {
         h->err = "task exceeds the batch or stack capacity";
         return DQC_E_CAPACITY;
     }
@@ -1880,17 +1973,23 @@ dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, 
 dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, int32_t* meta, uint32_t* records) {
     if (!h || !ntasks || !nrecords) return DQC_E_ARG;
     hipSetDevice(h->device);
-    *ntasks = (int64_t)h->st_t;
-    *nrecords = (int64_t)h->st_v;
-    if (!meta || !records || h->st_t == 0) return DQC_OK;
-    std::vector<TaskDesc> d(h->st_t);
-    if (!hip_ok(h, hipMemcpy(d.data(), h->d_sdesc, sizeof(TaskDesc) * h->st_t, hipMemcpyDeviceToHost), "stack read") ||
-        !hip_ok(h, hipMemcpy(records, h->d_svtx, sizeof(dqc::TaskVertex) * h->st_v, hipMemcpyDeviceToHost), "stack read"))
+    // the main stack, then the deferred region (records follow the main stack's)
+    *ntasks = (int64_t)(h->st_t + h->def_t);
+    *nrecords = (int64_t)(h->st_v + h->def_v);
+    if (!meta || !records || *ntasks == 0) return DQC_OK;
+    std::vector<TaskDesc> d((size_t)*ntasks);
+    const unsigned long long dt0 = h->cfg.spill_tasks - h->def_t, dv0 = h->cfg.spill_vertices - h->def_v;
+    if ((h->st_t && !hip_ok(h, hipMemcpy(d.data(), h->d_sdesc, sizeof(TaskDesc) * h->st_t, hipMemcpyDeviceToHost), "stack read")) ||
+        (h->st_v && !hip_ok(h, hipMemcpy(records, h->d_svtx, sizeof(dqc::TaskVertex) * h->st_v, hipMemcpyDeviceToHost), "stack read")) ||
+        (h->def_t && !hip_ok(h, hipMemcpy(d.data() + h->st_t, h->d_sdesc + dt0, sizeof(TaskDesc) * h->def_t, hipMemcpyDeviceToHost), "stack read")) ||
+        (h->def_v && !hip_ok(h, hipMemcpy(records + 6 * h->st_v, h->d_svtx + dv0, sizeof(dqc::TaskVertex) * h->def_v, hipMemcpyDeviceToHost), "stack read")))
         return DQC_E_HIP;
-    for (unsigned long long i = 0; i < h->st_t; ++i) {
-        int32_t* m = meta + 8 * i;
-        m[0] = (int32_t)d[i].off; m[1] = d[i].tot; m[2] = d[i].nm; m[3] = d[i].mode;
+    for (unsigned long long i = 0; i < h->st_t + h->def_t; ++i) {
+        int32_t* m = meta + 9 * i;
+        const unsigned long long off = i < h->st_t ? d[i].off : d[i].off - dv0 + h->st_v;
+        m[0] = (int32_t)off; m[1] = d[i].tot; m[2] = d[i].nm; m[3] = d[i].mode;
         m[4] = (int32_t)d[i].uid; m[5] = (int32_t)d[i].parent; m[6] = d[i].depth; m[7] = d[i].flags;
+        m[8] = d[i].ncov;
     }
     return DQC_OK;
 }
@@ -1900,6 +1999,10 @@ dqc_status dqc_reset(dqc_handle* h) {
     hipSetDevice(h->device);
     h->st_t = 0;
     h->st_v = 0;
+    h->def_t = 0;
+    h->def_v = 0;
+    h->peak_t = 0;
+    h->peak_v = 0;
     h->root_pending = false;
     h->host_cliques.clear();
     GCtr g{};
@@ -1926,36 +2029,62 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
 // budget of pool_vertices / 16 piece records per batch (pieces copy the task's records
 // and run all of its branches at once, so the budget bounds the stack growth).
 static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* popped) {
-    unsigned long long m = std::min<unsigned long long>(h->st_t, h->cfg.pool_tasks);
-    std::vector<TaskDesc> tail(m);
-    if (!hip_ok(h, hipMemcpy(tail.data(), h->d_sdesc + (h->st_t - m), sizeof(TaskDesc) * m, hipMemcpyDeviceToHost), "stack read"))
+    // newest tasks of the main stack, or of the deferred region when the main stack is empty
+    const bool from_main = h->st_t > 0;
+    const unsigned long long avail = from_main ? h->st_t : h->def_t;
+    const unsigned long long m = std::min<unsigned long long>(avail, h->cfg.pool_tasks);
+    const unsigned long long lo = from_main ? h->st_t - m : h->cfg.spill_tasks - h->def_t;  // lowest index read
+    std::vector<TaskDesc> rd(m);
+    if (!hip_ok(h, hipMemcpy(rd.data(), h->d_sdesc + lo, sizeof(TaskDesc) * m, hipMemcpyDeviceToHost), "stack read"))
         return DQC_E_HIP;
-    const unsigned long long vend = h->st_v;
-    size_t skip = 0;  // keep the newest tasks whose records fit the batch buffer
-    while (skip < m && vend - tail[skip].off > h->cfg.pool_vertices) ++skip;
-    if (skip == m) { h->err = "a task does not fit the batch buffer (raise pool_vertices)"; return DQC_E_CAPACITY; }
-    // ... and whose children fit the stack: a task writes at most one child per step, of
-    // at most its own size, so the batch keeps the newest tasks whose worst-case output
-    // fits the space left once they are popped (always at least one task)
-    {
-        size_t keep = m;  // index of the oldest task kept
-        unsigned long long recs = 0, descs = 0;
-        for (size_t i = m; i-- > skip;) {
-            const TaskDesc& d = tail[i];
-            const unsigned long long steps = (unsigned long long)std::max(0, (d.flags & TF_PIECE) ? d.iters : d.tot - d.nm);
-            const unsigned long long free_v = h->cfg.spill_vertices - d.off, free_t = h->cfg.spill_tasks - (h->st_t - m + i);
-            recs += steps * (unsigned long long)d.tot;
-            descs += steps + MAX_PIECES;
-            if (keep < m && (recs > free_v || descs > free_t)) break;
-            keep = i;
-        }
-        skip = keep;
+    // the j-th newest task, and the records from the newest task through it
+    auto at = [&](unsigned long long j) -> const TaskDesc& { return from_main ? rd[m - 1 - j] : rd[j]; };
+    auto span = [&](const TaskDesc& d) {
+        return from_main ? h->st_v - d.off : d.off + (unsigned long long)d.tot - (h->cfg.spill_vertices - h->def_v);
+    };
+    if (span(at(0)) > h->cfg.pool_vertices) {
+        h->err = "a task does not fit the batch buffer (raise pool_vertices)";
+        return DQC_E_CAPACITY;
     }
-    const unsigned long long first = h->st_t - m + skip, cnt = m - skip, v0 = tail[skip].off;
+    unsigned long long cnt = 1;  // keep the newest tasks whose records fit the batch buffer
+    while (cnt < m && span(at(cnt)) <= h->cfg.pool_vertices) ++cnt;
+    // (the stack cannot overflow from the batch's children: a workgroup that finds the
+    // stack nearly full defers the rest of its task; see stack_low). The launch runs as
+    // many workgroups as the stack has room for one child each; with no room for the
+    // batch, the newest task alone is tried.
+    auto room = [&](unsigned long long c) {
+        unsigned long long mt = 0;
+        for (unsigned long long j = 0; j < c; ++j) mt = std::max<unsigned long long>(mt, (unsigned long long)at(j).tot);
+        h->batch_maxtot = std::max<unsigned long long>(mt, 1);
+        const unsigned long long room_v = (h->cfg.spill_vertices - h->st_v - h->def_v) / h->batch_maxtot;
+        const unsigned long long room_t = h->cfg.spill_tasks - h->st_t - h->def_t;
+        h->batch_grid = std::min<unsigned long long>((unsigned long long)h->launch_wg, std::min(room_v, room_t));
+    };
+    room(cnt);
+    if (h->batch_grid == 0) {
+        cnt = 1;
+        room(cnt);
+    }
+    if (h->batch_grid == 0) {
+        char st[256];
+        std::snprintf(st, sizeof st, " [main %llu tasks / %llu records, deferred %llu / %llu, capacity %llu / %llu, "
+                      "newest task %d records]", h->st_t, h->st_v, h->def_t, h->def_v,
+                      (unsigned long long)h->cfg.spill_tasks, (unsigned long long)h->cfg.spill_vertices, at(0).tot);
+        h->err = std::string("task stack full: no room left for the newest task's children (raise spill_tasks / "
+                             "spill_vertices)") + st;
+        return DQC_E_CAPACITY;
+    }
+    // under stack pressure (fewer workgroups than resident), two tasks per workgroup: a
+    // full batch would mostly be deferred untouched
+    if (h->batch_grid < (unsigned long long)h->launch_wg) cnt = std::min(cnt, 2 * h->batch_grid);
+    // the batch: descriptor indices [first, first + cnt), records [v0, v0 + recs)
+    const unsigned long long first = from_main ? h->st_t - cnt : lo;
+    const TaskDesc* b = from_main ? rd.data() + (m - cnt) : rd.data();
+    const unsigned long long v0 = b[0].off, recs = b[cnt - 1].off + (unsigned long long)b[cnt - 1].tot - v0;
+    h->batch_pend = (cnt << CNT_SHIFT) | recs;
     if (sched == DQC_SCHED_FIFO) {
         refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
     } else {
-        const TaskDesc* b = tail.data() + skip;
         h->hcost.resize(cnt);
         dqc_status rc = batch_costs(h, b, cnt, first, sched, h->hcost.data());
         if (rc != DQC_OK) return rc;
@@ -1963,8 +2092,9 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         for (uint32_t i = 0; i < cnt; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h->hcost[x] > h->hcost[y]; });
         h->hbatch.resize(cnt);
+        const unsigned long long used = h->st_v + h->def_v;
         unsigned long long budget = h->cfg.pool_vertices / 16;
-        if (h->st_v + budget > h->cfg.spill_vertices) budget = h->cfg.spill_vertices > h->st_v ? (h->cfg.spill_vertices - h->st_v) / 2 : 0;
+        if (used + budget > h->cfg.spill_vertices) budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 2 : 0;
         for (unsigned long long i = 0; i < cnt; ++i) {
             TaskDesc d = b[order[i]];
             d.off -= v0;
@@ -1986,8 +2116,13 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, nullptr, h->d_bvtx, 0, 0);
     }
     if (!hip_ok(h, hipGetLastError(), "pop launch")) return DQC_E_HIP;
-    h->st_t = first;
-    h->st_v = v0;
+    if (from_main) {
+        h->st_t = first;
+        h->st_v = v0;
+    } else {
+        h->def_t -= cnt;
+        h->def_v -= recs;
+    }
     *popped = cnt;
     return DQC_OK;
 }
@@ -2010,10 +2145,10 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
             !hip_ok(h, hipMemcpyAsync(h->hpred.data(), h->d_cost, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream), "cost read") ||
             !hip_ok(h, hipStreamSynchronize(h->stream), "cost kernel"))
             return DQC_E_HIP;
-        for (size_t i = 0; i < n; ++i) {
+        for (size_t i = 0; i < n; ++i) {  // predicted branches x task size
             const double steps = (double)std::max(1, b[i].tot - b[i].nm);
             const double share = (b[i].flags & TF_PIECE) ? std::min(1.0, (double)b[i].iters / steps) : 1.0;
-            cost[i] = std::expm1((double)std::min(h->hpred[i], 60.f)) * share;
+            cost[i] = std::expm1((double)std::min(h->hpred[i], 60.f)) * share * (double)b[i].tot;
         }
         return DQC_OK;
     }
@@ -2061,6 +2196,8 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.log_cap = h->log_cap;
     P.kloc = h->kloc;
     P.gbase = h->d_gbase;
+    P.max_depth = (uint32_t)std::max(0, h->cfg.max_depth);
+    P.maxtot = std::max<unsigned long long>(h->batch_maxtot, 1);
     return P;
 }
 
@@ -2071,6 +2208,10 @@ static void fold_stats(dqc_handle* h, const GCtr& g) {
     h->st.raw_cliques = (int64_t)g.cliques + (int64_t)h->host_cliques.size();
     h->st.bytes_scanned = (int64_t)g.bytes;
     h->st.slow_sorts = (int64_t)g.slow_sorts;
+    h->st.truncated_children = (int64_t)g.truncated;
+    h->st.stack_pushbacks = (int64_t)g.pushbacks;
+    h->st.peak_stack_records = (int64_t)h->peak_v;
+    h->st.peak_stack_tasks = (int64_t)h->peak_t;
     for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
 }
 
@@ -2080,8 +2221,12 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     q.next_task = 0;
     q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
     q.root_first = ~0ull;
+    q.pend = P.split_s ? 0 : h->batch_pend;
+    q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
-    const int grid = (int)std::min<unsigned long long>((unsigned long long)h->launch_wg, ntask);
+    unsigned long long gmax = (unsigned long long)h->launch_wg;
+    if (!P.split_s) gmax = std::min(gmax, h->batch_grid);
+    const int grid = (int)std::min<unsigned long long>(gmax, ntask);
     hipEventRecord(h->ev0, h->stream);
     if (h->big) expand_kernel<true><<<grid, NT, 0, h->stream>>>(P);
     else expand_kernel<false><<<grid, NT, 0, h->stream>>>(P);
@@ -2098,13 +2243,21 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     h->host_launches++;
     fold_stats(h, g);
     if (g.error) {
-        h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
-               : (g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
-               : "a task has more members than the member histograms hold";
+        char st[200];
+        std::snprintf(st, sizeof st, " [top %llu/%llu, request %llu, state %llx, pending %llu/%llu]",
+                      g.spill_state[0] >> CNT_SHIFT, g.spill_state[0] & SIZE_MASK, g.spill_state[1], g.spill_state[2],
+                      g.spill_state[3] >> CNT_SHIFT, g.spill_state[3] & SIZE_MASK);
+        h->err = (g.error & ERR_SPILL) ? std::string("task stack full (raise spill_tasks / spill_vertices)") + st
+               : std::string((g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
+                                                     : "a task has more members than the member histograms hold");
         return (g.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
     }
     h->st_t = q.sp_alloc >> CNT_SHIFT;
     h->st_v = q.sp_alloc & SIZE_MASK;
+    h->def_t = q.def_alloc >> CNT_SHIFT;
+    h->def_v = q.def_alloc & SIZE_MASK;
+    h->peak_t = std::max(h->peak_t, h->st_t + h->def_t);
+    h->peak_v = std::max(h->peak_v, h->st_v + h->def_v);
     h->last_root_first = q.root_first;
     h->max_depth = std::max(h->max_depth, h->st_t);
     return DQC_OK;
@@ -2172,7 +2325,7 @@ static dqc_status split_root(dqc_handle* h) {
 // expand. Nothing in it waits for the host.
 static void enqueue_step(dqc_handle* h, int32_t sched) {
     PopParams pp{h->d_sdesc, h->d_q, h->d_g, h->cfg.pool_tasks, h->cfg.pool_vertices, h->cfg.spill_tasks,
-                 h->cfg.spill_vertices};
+                 h->cfg.spill_vertices, (unsigned long long)h->launch_wg};
     pop_kernel<<<1, POP_T, 0, h->stream>>>(pp);
     const uint32_t* perm = nullptr;
     const uint8_t* mark = nullptr;
@@ -2188,7 +2341,8 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
         mark = h->d_mark;
     }
     gather_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, h->d_q, perm, mark, h->d_bdesc, h->d_bvtx);
-    const KParams P = make_params(h, ~0ull);
+    KParams P = make_params(h, ~0ull);
+    P.maxtot = ~0ull;  // read from the device-resident pop
     if (h->big) expand_kernel<true><<<h->launch_wg, NT, 0, h->stream>>>(P);
     else expand_kernel<false><<<h->launch_wg, NT, 0, h->stream>>>(P);
 }
@@ -2203,6 +2357,7 @@ static dqc_status resident_run(dqc_handle* h, int32_t sched, int64_t max_steps, 
         return DQC_E_HIP;
     QCtr q{};
     q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
+    q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
     q.root_first = ~0ull;
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
     hipGraphExec_t& ex = h->gexec[sched];
@@ -2235,6 +2390,10 @@ static dqc_status resident_run(dqc_handle* h, int32_t sched, int64_t max_steps, 
         fold_stats(h, g);
         h->st_t = q.sp_alloc >> CNT_SHIFT;
         h->st_v = q.sp_alloc & SIZE_MASK;
+        h->def_t = q.def_alloc >> CNT_SHIFT;
+        h->def_v = q.def_alloc & SIZE_MASK;
+        h->peak_t = std::max(h->peak_t, h->st_t + h->def_t);
+        h->peak_v = std::max(h->peak_v, h->st_v + h->def_v);
         if (g.error) {
             h->err = (g.error & ERR_SPILL) ? "task stack full (raise spill_tasks / spill_vertices)"
                    : (g.error & ERR_CLIQUES) ? "clique store full (raise clique_sets / clique_ids)"
@@ -2242,7 +2401,7 @@ static dqc_status resident_run(dqc_handle* h, int32_t sched, int64_t max_steps, 
                    : "a task has more members than the member histograms hold";
             return (g.error & ERR_MEMBERS) ? DQC_E_UNSUPPORTED : DQC_E_CAPACITY;
         }
-        if (h->st_t == 0) break;
+        if (h->st_t == 0 && h->def_t == 0) break;
     }
     return DQC_OK;
 }
@@ -2256,7 +2415,7 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
     const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
     auto last_report = t0;
     int64_t done = 0;
-    while (h->st_t > 0 && (max_launches < 0 || done < max_launches)) {
+    while ((h->st_t > 0 || h->def_t > 0) && (max_launches < 0 || done < max_launches)) {
         ++done;
         if (h->st_t == 1 && h->root_pending) {
             h->root_pending = false;
@@ -2283,14 +2442,14 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
             }
         }
     }
-    if (h->st_t == 0) {  // everything done: final counters
+    if (h->st_t == 0 && h->def_t == 0) {  // everything done: final counters
         GCtr g{};
         if (!hip_ok(h, hipMemcpy(&g, h->d_g, sizeof g, hipMemcpyDeviceToHost), "ctr read")) return DQC_E_HIP;
         fold_stats(h, g);
     }
     auto t1 = std::chrono::steady_clock::now();
     h->st.expand_wall_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    *finished = h->st_t == 0;
+    *finished = h->st_t == 0 && h->def_t == 0;
     return DQC_OK;
 }
 

@@ -6,6 +6,8 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -104,6 +106,9 @@ void build_twohop(Graph& g, int threads) {
     const int32_t chunk = 1024;
     const int32_t nchunks = (n + chunk - 1) / chunk;
     std::vector<std::vector<int32_t>> chunk_lists(nchunks);
+    // DQC_PROGRESS: one line per tenth of the chunks done (long builds on large graphs)
+    const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
+    std::atomic<int32_t> chunks_done{0};
 #pragma omp parallel num_threads(threads)
     {
         // state[x] bit layout, per vertex being processed (reset after each vertex):
@@ -113,6 +118,11 @@ void build_twohop(Graph& g, int threads) {
         std::vector<int32_t> B, O, I, nO, nI, lst, touched;
 #pragma omp for schedule(dynamic, 1)
         for (int32_t ch = 0; ch < nchunks; ++ch) {
+            if (progress) {
+                const int32_t d = chunks_done.fetch_add(1);
+                if (nchunks >= 10 && d % (nchunks / 10) == 0)
+                    std::fprintf(stderr, "dqc: two-hop lists %d/%d chunks\n", d, nchunks);
+            }
             std::vector<int32_t>& dst = chunk_lists[ch];
             int32_t v0 = ch * chunk, v1 = std::min(n, v0 + chunk);
             for (int32_t v = v0; v < v1; ++v) {
@@ -236,6 +246,10 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
                      Level0& res) {
     const int32_t n = g.n;
     const uint32_t tho = (uint32_t)md_out[min_size], thi = (uint32_t)md_in[min_size];
+    const bool progress = std::getenv("DQC_PROGRESS") != nullptr;  // phase lines for large graphs
+    auto note = [&](const char* what, long long x) {
+        if (progress) std::fprintf(stderr, "dqc: level 0: %s %lld\n", what, x);
+    };
     std::vector<L0V> v((size_t)n);
     std::vector<int32_t> alive, dropped;
     for (int32_t i = 0; i < n; ++i) {
@@ -263,7 +277,9 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
             else { v[x].label = -1; dropped.push_back(x); }
         }
         alive.swap(next);
+        note("peel round, survivors", (long long)alive.size());
     }
+    note("initial filter and peeling done, survivors", (long long)alive.size());
     // cascade the last dropped set through the survivors
     int32_t k = (int32_t)alive.size();
     for (size_t h = 0; h < dropped.size(); ++h) {
@@ -323,10 +339,13 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
             for (int32_t r = 0; r < k; ++r) (*len_out)[r] = (uint32_t)rows[r].size();
         }
     };
+    note("cascade, cover and order done, condensed vertices", k);
     std::vector<uint32_t> l2c, odc, idc;
     restrict_csr(g.two, c.two, &l2c);
+    note("condensed two-hop entries", (long long)c.two.nbr.size());
     restrict_csr(g.out, c.out, &odc);
     restrict_csr(g.in, c.in, &idc);
+    note("condensed arcs", (long long)c.out.nbr.size());
     c.m = (int64_t)c.out.nbr.size();
     res.task.resize((size_t)k);
     for (int32_t r = 0; r < k; ++r) {

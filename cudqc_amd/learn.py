@@ -1,10 +1,14 @@
-"""Training data and training of the subtree-cost model behind DQC_SCHED_LEARNED.
+"""Training data and training of the cost model behind DQC_SCHED_LEARNED.
 
 Task logs come from the enumerator itself (dqc_set_task_log): one record per expanded
-task with its features and the branches it ran. A task's subtree cost is the sum of
-branches over the task and all its descendants; it is known exactly once no descendant
-is left on the task stack, so only those tasks become training samples. The target is
-log(1 + subtree branches), the quantity the model predicts.
+task with its features and the branches it ran. Two targets:
+  own      log(1 + branches of the task's own loop): what a task costs inside the launch
+           that runs it, the quantity batch ordering and splitting balance (default; every
+           logged task is a sample);
+  subtree  log(1 + branches over the task and all its descendants): known exactly once no
+           descendant is left on the task stack, so only those tasks become samples.
+The schedule multiplies the prediction by the task size (each branch scans lists of the
+task's vertices).
 
 Usage (on a GPU): python -m cudqc_amd.learn --out cudqc_amd/models/cost_mlp.f32
 """
@@ -48,7 +52,15 @@ def subtree_samples(records, remaining):
     return x, t
 
 
-def collect(graph, gammas, steps, log_capacity=1 << 23, sched=dq.SCHED_FIFO):
+def own_samples(records):
+    """Features and log1p(own branches) of every logged task (pieces excluded)."""
+    keep = [r for r in records if not (r.flags & 1)]
+    x = dq.task_features(keep)
+    t = np.log1p(np.array([r.branches for r in keep], dtype=np.float64)).astype(np.float32)
+    return x, t
+
+
+def collect(graph, gammas, steps, log_capacity=1 << 23, sched=dq.SCHED_FIFO, target="own"):
     """Runs `steps` expansion batches on one graph with the task log on; returns samples."""
     go, gi, mn = gammas
     cfg = dq.Config()
@@ -63,7 +75,7 @@ def collect(graph, gammas, steps, log_capacity=1 << 23, sched=dq.SCHED_FIFO):
         if dropped:
             raise RuntimeError(f"task log overflow ({dropped} records dropped)")
         remaining = eng.task_ids()
-    return subtree_samples(recs, remaining)
+    return own_samples(recs) if target == "own" else subtree_samples(recs, remaining)
 
 
 def train(x, t, epochs=30, batch=1024, lr=2e-3, seed=1):
@@ -84,12 +96,13 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--target", choices=("own", "subtree"), default="own")
     args = ap.parse_args()
     if not os.path.exists(args.graph):
         dq.generate(2, args.graph)
     t0 = time.time()
-    x, t = collect(args.graph, dq.WORKLOADS[2], args.steps)
-    print(f"samples {len(x)} (complete subtrees), target mean {t.mean():.3f} var {t.var():.3f}, "
+    x, t = collect(args.graph, dq.WORKLOADS[2], args.steps, target=args.target)
+    print(f"samples {len(x)} ({args.target} cost), target mean {t.mean():.3f} var {t.var():.3f}, "
           f"{time.time() - t0:.1f}s", flush=True)
     rng = np.random.default_rng(0)
     perm = rng.permutation(len(x))

@@ -46,6 +46,9 @@ typedef struct {
     int32_t split_chunk;       /* least branches per piece; 0 = 32                    */
     int32_t schedule;          /* DQC_SCHED_* used by dqc_search_file and by
                                   DQC_SCHED_DEFAULT; 0 = DQC_SCHED_SIZE               */
+    int32_t max_depth;         /* > 0: depth-bounded enumeration: children deeper than
+                                  this (the level-0 task has depth 0) are counted in
+                                  truncated_children and not expanded; 0 = no bound   */
     int32_t resident_steps;    /* > 0: batches are popped, ordered and expanded on the
                                   device, resident_steps per replay of one captured HIP
                                   graph, with no host round trip between them;
@@ -64,6 +67,11 @@ typedef struct {
     double expand_wall_ms;     /* host wall time of dqc_run                           */
     int32_t n_condensed;       /* vertices after level-0 condensing                   */
     int32_t workgroups;
+    int64_t truncated_children;/* children not expanded (depth-bounded runs)          */
+    int64_t stack_pushbacks;   /* tasks whose remaining steps were deferred because
+                                  the task stack was nearly full                      */
+    int64_t peak_stack_tasks;  /* largest task-stack use seen between launches        */
+    int64_t peak_stack_records;
 } dqc_stats;
 
 /* One record per expanded task (or piece of a split task): the task's id, its parent's
@@ -135,9 +143,9 @@ dqc_status dqc_run(dqc_handle* h, int32_t sched);
 dqc_status dqc_push_task(dqc_handle* h, const uint32_t* records, int32_t nvert, int32_t nm, int32_t ncov,
                          int32_t mode);
 
-/* Reads the task stack: meta gets 8 int32 per task {record offset, size, members, mode,
- * id, parent id, depth, flags} and records the 6-uint32 records. NULL buffers query the
- * sizes. */
+/* Reads the task stack: meta gets 9 int32 per task {record offset, size, members, mode,
+ * id, parent id, depth, flags, covered} and records the 6-uint32 records. NULL buffers
+ * query the sizes. */
 dqc_status dqc_fetch_tasks(dqc_handle* h, int64_t* ntasks, int64_t* nrecords, int32_t* meta, uint32_t* records);
 
 /* Empties the task stack and the clique store (the graph and tables stay). */
@@ -150,9 +158,10 @@ dqc_status dqc_set_task_log(dqc_handle* h, int64_t capacity);
  * capacity. */
 dqc_status dqc_fetch_task_log(dqc_handle* h, int64_t* count, int64_t* dropped, dqc_task_log* records);
 
-/* Subtree-cost model of the learned schedule (no counterpart in the reference): an MLP
+/* Cost model of the learned schedule (no counterpart in the reference): an MLP
  * 7 -> 64 -> 64 -> 1 (ReLU, fp32) over the features of dqc_task_features, predicting
- * log(1 + branches in the task's subtree). Parameters, DQC_MLP_PARAMS floats: W1[64][7],
+ * log(1 + branches) of a task: its own loop's (the shipped model; the schedule weighs it
+ * by the task size) or its whole subtree's. Parameters, DQC_MLP_PARAMS floats: W1[64][7],
  * b1[64], W2[64][64], b2[64], W3[64], b3. Forward, gradient and training run as HIP
  * kernels on the current device; all buffers are host memory. */
 #define DQC_MLP_IN 7

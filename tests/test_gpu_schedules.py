@@ -14,9 +14,10 @@ from tests.conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 
-def run_golden(dqc, manifest, tmp_path, cfg, model=None):
+def run_golden(dqc, manifest, tmp_path, cfg, model=None, pushbacks=None):
     bad = []
     for c in manifest["cases"]:
+        print("case", c["name"], flush=True)
         path = os.path.join(GOLDEN, c["graph"])
         with dqc.Engine(cfg) as eng:
             eng.prepare(path, c["out_gamma"], c["in_gamma"], c["min_size"])
@@ -25,6 +26,8 @@ def run_golden(dqc, manifest, tmp_path, cfg, model=None):
             eng.run()
             st = eng.stats()
             cl = eng.cliques()
+        if pushbacks is not None:
+            pushbacks.append(st.stack_pushbacks)
         mx = str(tmp_path / (c["name"] + ".max"))
         if os.path.exists(mx):
             os.remove(mx)
@@ -93,3 +96,37 @@ def test_learned_gain_on_batch_workload(dqc, tmp_path):
     fifo = min(out["fifo"][0], out["fifo2"][0])
     learned = min(out["learned"][0], out["learned2"][0])
     print(f"learned gain over FIFO (wall): {fifo / learned:.3f}x", flush=True)
+
+
+def test_backpressure_below_peak(dqc, manifest, tmp_path, monkeypatch):
+    """Each case is run once with the default stack to measure its peak stack use, then
+    with a stack of three quarters of that peak: workgroups that find it nearly full defer
+    the rest of their task (it runs once the main stack is empty). Every run either gives
+    the golden bytes and counts or stops cleanly with DQC_E_CAPACITY, and the deferral
+    path ran in completed runs."""
+    monkeypatch.setenv("DQC_SERIAL_ROOT", "1")  # level 0 through the task loop (checked)
+    cases = []
+    for c in manifest["cases"]:
+        cfg = config(dqc, dqc.SCHED_FIFO)
+        with dqc.Engine(cfg) as eng:
+            eng.prepare(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"])
+            eng.run()
+            st = eng.stats()
+        if st.peak_stack_records >= 4 * max(c["n_condensed"], 1) * 8:
+            cases.append((c, st.peak_stack_records, st.peak_stack_tasks))
+    assert len(cases) >= 5
+    done, bad, clean = [], [], 0
+    for c, pv, pt in cases:
+        cfg = config(dqc, dqc.SCHED_FIFO)
+        cfg.spill_vertices = pv * 3 // 4
+        cfg.spill_tasks = max(pt * 3 // 4, 64)
+        pb = []
+        try:
+            bad += run_golden(dqc, {"cases": [c]}, tmp_path, cfg, pushbacks=pb)
+            done.append((c["name"], pb[0]))
+        except dqc.DQCError as e:  # a stack too small for the search fails cleanly
+            assert "DQC_E_CAPACITY" in str(e), e
+            clean += 1
+    print(f"completed (case, deferred): {done}; clean capacity errors: {clean}", flush=True)
+    assert not bad, bad
+    assert any(n > 0 for _, n in done)
