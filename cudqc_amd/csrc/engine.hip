@@ -48,6 +48,11 @@ constexpr int KMAX = KW * 32;
 #define DQC_SORT_N 256
 #endif
 constexpr int SORT_N = DQC_SORT_N;  // LDS sort capacity (larger children sort in HBM)
+#ifndef DQC_CHUNKED_SORT
+#define DQC_CHUNKED_SORT 0
+#endif
+// children above SORT_N: bitonic sort with the short steps on NT-key chunks (needs SORT_N == NT)
+constexpr bool CHUNKED_SORT = DQC_CHUNKED_SORT != 0 && SORT_N >= NT;
 constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
 #ifndef DQC_SCAN_UNR
 #define DQC_SCAN_UNR 8
@@ -148,6 +153,7 @@ struct KParams {
     TaskLog* log;               // task log (null = off)
     unsigned long long log_cap;
     int32_t kloc;               // largest vertex count of one graph (scratch arrays, bitmaps)
+    int32_t sum_shift;          // least log2 ids per live-summary bit (HBM-bitmap variant)
     uint32_t max_depth;         // children deeper than this are counted, not pushed (0 = no bound)
     unsigned long long maxtot;  // largest task of the batch (~0ull: q->maxtot, device-resident steps)
     const int32_t* gbase;       // batched graphs: first id of each graph (multiples of 32); null = one graph
@@ -158,6 +164,9 @@ struct KParams {
 // workgroup's HBM area. Either way they are all-zero between tasks: each branch and
 // task clears the words of the vertices it set.
 constexpr int NBITMAPS = 6;
+// LDS words of the live-set summary of the HBM-bitmap variant (it takes the LDS the
+// other variant gives its bitmaps): one bit per id up to 98,304 ids, per 2^s ids above
+constexpr int SUMW = 3072;
 struct Shared {
     unsigned long long sortk[SORT_N];
     int32_t ho[HMAX + 2], hi[HMAX + 2], so[HMAX + 2], si[HMAX + 2];
@@ -198,8 +207,26 @@ struct W {
     uint32_t* vbase;   // the workgroup's id-indexed value arrays (graph-relative)
     int gb;            // first id of the current task's graph (a multiple of 32)
     uint32_t cur_gid;
+    // HBM-bitmap variant: a summary of the parent-live bitmap in LDS, one bit per 2^gsh
+    // ids, set with the live bits and cleared with them at the end of the task (removals
+    // leave it set). Every id in any of the six bitmaps is live in the parent, so an id
+    // whose summary bit is clear is in none of them and bt answers from LDS without the
+    // HBM read: the common case, since few neighbours of a vertex are in the task.
+    uint32_t* lsum;  // null in the LDS-bitmap variant
+    int gsh;
+    __device__ __forceinline__ bool maybe_live(int id) const {
+        const int g = (id - gb) >> gsh;
+        return (lsum[g >> 5] >> (g & 31)) & 1u;
+    }
+    __device__ __forceinline__ void mark_live(int id) const {
+        const int g = (id - gb) >> gsh;
+        atomicOr(&lsum[g >> 5], 1u << (g & 31));
+    }
     // graph-relative bit tests on the id bitmaps (gb is a multiple of 32)
-    __device__ __forceinline__ bool bt(const uint32_t* bm, int id) const { return (bm[(id - gb) >> 5] >> (id & 31)) & 1u; }
+    __device__ __forceinline__ bool bt(const uint32_t* bm, int id) const {
+        if (lsum && !maybe_live(id)) return false;
+        return (bm[(id - gb) >> 5] >> (id & 31)) & 1u;
+    }
     __device__ __forceinline__ void bs(uint32_t* bm, int id) const { atomicOr(&bm[(id - gb) >> 5], 1u << (id & 31)); }
     __device__ __forceinline__ void bcl(uint32_t* bm, int id) const { atomicAnd(&bm[(id - gb) >> 5], ~(1u << (id & 31))); }
     // points the id-indexed arrays at graph `gid` of a batched run
@@ -628,6 +655,57 @@ struct W {
         bytes += 4ull * len / NT + 1;
     }
 
+    // Bitonic sort of Np (a power of two above NT) 64-bit keys in the HBM sort area. The
+    // steps whose partner distance j is below NT run on 256-key chunks held one key per
+    // thread: j < 64 pairs lanes of one wavefront (a lane shuffle, no barrier), j = 64 and
+    // 128 exchange through the LDS sort buffer. Only the steps with j >= NT sweep the
+    // whole area, so the area is read and written 1 + log2(Np / NT) * (log2(Np / NT) + 3) / 2
+    // times instead of log2(Np) * (log2(Np) + 1) / 2.
+    __device__ __forceinline__ unsigned long long bitonic_partner(unsigned long long x, int j) {
+        if (j < 64) {
+            const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)x, j);
+            const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(x >> 32), j);
+            return ((unsigned long long)hi << 32) | lo;
+        }
+        __syncthreads();
+        S->sortk[tid] = x;
+        __syncthreads();
+        return S->sortk[tid ^ j];
+    }
+    // steps j = jmax .. 1 of stages kmin .. kmax (kmin = kmax after the first pass) on
+    // every chunk; each compare-exchange keeps min or max by lane position and direction
+    __device__ __forceinline__ void bitonic_chunks(unsigned long long* sk, int Np, int kmin, int kmax) {
+        for (int c = 0; c < Np; c += NT) {
+            const int gi = c + tid;
+            unsigned long long x = sk[gi];
+            for (int k = kmin; k <= kmax; k <<= 1)
+                for (int j = min(k >> 1, NT >> 1); j > 0; j >>= 1) {
+                    const unsigned long long y = bitonic_partner(x, j);
+                    const bool keep_min = ((tid & j) == 0) == ((gi & k) == 0);
+                    x = keep_min ? (x < y ? x : y) : (x < y ? y : x);
+                }
+            sk[gi] = x;
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ void sort_hbm_keys(unsigned long long* sk, int Np) {
+        bitonic_chunks(sk, Np, 2, NT);
+        for (int k = 2 * NT; k <= Np; k <<= 1) {
+            for (int j = k >> 1; j >= NT; j >>= 1) {
+                for (int i = tid; i < Np; i += NT) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned long long x = sk[i], y = sk[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+            bitonic_chunks(sk, Np, k, k);
+        }
+    }
+
     // ---------------------------------------------------------------- sort + write
     // Orders members then candidates by the enumeration order of reference
     // inc/device_kernels.hpp:31-80 (min(mem) desc, min(can) desc, lvl2adj desc, id desc)
@@ -707,18 +785,20 @@ struct W {
                 sk[i] = key;
             }
             __syncthreads();
-            for (int k = 2; k <= Np; k <<= 1)
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = tid; i < Np; i += NT) {
-                        int ixj = i ^ j;
-                        if (ixj > i) {
-                            unsigned long long x = sk[i], y = sk[ixj];
-                            bool up = (i & k) == 0;
-                            if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
+            if (CHUNKED_SORT && Np > SORT_N) sort_hbm_keys(sk, Np);
+            else
+                for (int k = 2; k <= Np; k <<= 1)
+                    for (int j = k >> 1; j > 0; j >>= 1) {
+                        for (int i = tid; i < Np; i += NT) {
+                            int ixj = i ^ j;
+                            if (ixj > i) {
+                                unsigned long long x = sk[i], y = sk[ixj];
+                                bool up = (i & k) == 0;
+                                if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
+                            }
                         }
+                        __syncthreads();
                     }
-                    __syncthreads();
-                }
             const unsigned long long idmask = bi == 0 ? 0ull : ((1ull << bi) - 1);
             for (int i = tid; i < N; i += NT) {
                 int x = (int)(idx_max - (uint32_t)(sk[i] & idmask));
@@ -1170,6 +1250,7 @@ struct W {
             pids[i] = x;
             pom[x] = r.out_mem; poc[x] = r.out_can; pim[x] = r.in_mem; pic[x] = r.in_can; pl2[x] = r.lvl2adj;
             bs(plive, x);
+            if (lsum) mark_live(x);
             if (i < nm) bs(pmem, x);
         }
         bytes += 24ull * T / NT + 1;
@@ -1222,10 +1303,18 @@ struct W {
             }
         }
         if (tid == 0) atomicSub(&P->q->pend, (1ull << CNT_SHIFT) | (unsigned long long)T);
+        if (lsum) __syncthreads();  // every test of the summary is done before it clears
         for (int i = tid; i < T; i += NT) {
             const int wd = (pids[i] - gb) >> 5;
             plive[wd] = 0; pmem[wd] = 0;
+            if (lsum) lsum[((pids[i] - gb) >> gsh) >> 5] = 0;  // every set bit came from a pid
         }
+        __syncthreads();
+    }
+    // sets every summary word (all ones: any id may be live; zero: the all-clear state)
+    __device__ __forceinline__ void fill_live_summary(uint32_t v) {
+        __syncthreads();
+        for (int i = tid; i < SUMW; i += NT) lsum[i] = v;
         __syncthreads();
     }
 };
@@ -1258,10 +1347,18 @@ void expand_kernel(KParams P) {
     uint32_t* bm;
     if constexpr (BIG) {
         bm = base + 22 * k;  // kept all-zero between tasks; zeroed when the graph is uploaded
+        __shared__ uint32_t lds_sum[SUMW];
+        for (int i = threadIdx.x; i < SUMW; i += NT) lds_sum[i] = 0;
+        w.lsum = lds_sum;
+        int gs = P.sum_shift;  // ids per summary bit: 2^gs, so that every graph-relative id has a bit
+        while (((unsigned long long)(P.kloc > 0 ? P.kloc - 1 : 0) >> gs) >= 32ull * SUMW) ++gs;
+        w.gsh = gs;
     } else {
         __shared__ uint32_t lds_bm[NBITMAPS * KW];
         bm = lds_bm;
         for (int i = threadIdx.x; i < NBITMAPS * KW; i += NT) lds_bm[i] = 0;
+        w.lsum = nullptr;
+        w.gsh = 0;
     }
     const int bw = BIG ? w.kw : KW;
     w.plive = bm; w.pmem = bm + bw; w.cmem = bm + 2 * bw; w.ccand = bm + 3 * bw; w.cL = bm + 4 * bw; w.ccrit = bm + 5 * bw;
@@ -1280,7 +1377,10 @@ void expand_kernel(KParams P) {
             w.set_graph(d.gid);
             w.cur_uid = d.uid;
             w.cur_depth = d.depth;
+            // level-0 branches keep no live bitmap (the live set is an id range)
+            if constexpr (BIG) w.fill_live_summary(~0u);
             w.branch_root(P.split_s[t], d.tot, P.in_vtx + d.off);
+            if constexpr (BIG) w.fill_live_summary(0u);
         } else {
             TaskDesc d = P.in_desc[t];
             w.run_task(d, P.in_vtx + d.off);
@@ -1608,6 +1708,7 @@ struct dqc_handle {
     int64_t extra_tasks = 0;
     bool root_pending = false;
     int profile = 0;
+    int sum_shift = 0;  // DQC_LIVE_SUMMARY_SHIFT: coarser live summary (tests of the HBM variant)
     unsigned long long phase[8] = {0};
     int64_t host_launches = 0;
     unsigned long long batch_maxtot = 0;  // largest task of the host-popped batch
@@ -1715,6 +1816,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
         return DQC_E_ARG;
     }
     h->profile = std::getenv("DQC_PHASES") != nullptr;
+    if (const char* e = std::getenv("DQC_LIVE_SUMMARY_SHIFT")) h->sum_shift = std::max(0, std::min(20, std::atoi(e)));
     *out = h;
     if (!hip_ok(h, hipSetDevice(h->device), "hipSetDevice")) return DQC_E_HIP;
     int cus = 0;
@@ -2180,6 +2282,7 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.min_size = h->min_size;
     P.host_addfail = h->cfg.host_addfail_check;
     P.profile = h->profile;
+    P.sum_shift = h->sum_shift;
     P.in_desc = h->d_bdesc; P.in_vtx = h->d_bvtx; P.in_ntask = ntask;
     P.sp_desc = h->d_sdesc; P.sp_vtx = h->d_svtx;
     P.sp_cap_t = h->cfg.spill_tasks; P.sp_cap_v = h->cfg.spill_vertices;

@@ -112,10 +112,15 @@ def test_split_everything_same_result(dqc, manifest, tmp_path):
     assert not bad, bad
 
 
-def test_hbm_bitmaps_same_result(dqc, manifest, tmp_path, monkeypatch):
+@pytest.mark.parametrize("summary_shift", [0, 4])
+def test_hbm_bitmaps_same_result(dqc, manifest, tmp_path, monkeypatch, summary_shift):
     """The kernel variant for condensed graphs above 16384 vertices keeps its id bitmaps in
-    HBM; forced on every golden case, it gives the same bytes and counts."""
+    HBM, behind a live-set summary in LDS (one bit per id up to 98,304 ids, per 2^s ids
+    above). Forced on every golden case, with an exact summary and with one bit per 16 ids,
+    it gives the same bytes and counts."""
     monkeypatch.setenv("DQC_HBM_BITMAPS", "1")
+    monkeypatch.setenv("DQC_LIVE_SUMMARY_SHIFT", str(summary_shift))
+    print("summary shift", summary_shift, flush=True)
     bad = []
     for c in manifest["cases"]:
         cnt, st, mx = run_case(dqc, c, tmp_path)
