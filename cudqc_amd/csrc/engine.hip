@@ -48,11 +48,6 @@ constexpr int KMAX = KW * 32;
 #define DQC_SORT_N 256
 #endif
 constexpr int SORT_N = DQC_SORT_N;  // LDS sort capacity (larger children sort in HBM)
-#ifndef DQC_CHUNKED_SORT
-#define DQC_CHUNKED_SORT 0
-#endif
-// children above SORT_N: bitonic sort with the short steps on NT-key chunks (needs SORT_N == NT)
-constexpr bool CHUNKED_SORT = DQC_CHUNKED_SORT != 0 && SORT_N >= NT;
 constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
 #ifndef DQC_SCAN_UNR
 #define DQC_SCAN_UNR 8
@@ -655,57 +650,6 @@ struct W {
         bytes += 4ull * len / NT + 1;
     }
 
-    // Bitonic sort of Np (a power of two above NT) 64-bit keys in the HBM sort area. The
-    // steps whose partner distance j is below NT run on 256-key chunks held one key per
-    // thread: j < 64 pairs lanes of one wavefront (a lane shuffle, no barrier), j = 64 and
-    // 128 exchange through the LDS sort buffer. Only the steps with j >= NT sweep the
-    // whole area, so the area is read and written 1 + log2(Np / NT) * (log2(Np / NT) + 3) / 2
-    // times instead of log2(Np) * (log2(Np) + 1) / 2.
-    __device__ __forceinline__ unsigned long long bitonic_partner(unsigned long long x, int j) {
-        if (j < 64) {
-            const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)x, j);
-            const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(x >> 32), j);
-            return ((unsigned long long)hi << 32) | lo;
-        }
-        __syncthreads();
-        S->sortk[tid] = x;
-        __syncthreads();
-        return S->sortk[tid ^ j];
-    }
-    // steps j = jmax .. 1 of stages kmin .. kmax (kmin = kmax after the first pass) on
-    // every chunk; each compare-exchange keeps min or max by lane position and direction
-    __device__ __forceinline__ void bitonic_chunks(unsigned long long* sk, int Np, int kmin, int kmax) {
-        for (int c = 0; c < Np; c += NT) {
-            const int gi = c + tid;
-            unsigned long long x = sk[gi];
-            for (int k = kmin; k <= kmax; k <<= 1)
-                for (int j = min(k >> 1, NT >> 1); j > 0; j >>= 1) {
-                    const unsigned long long y = bitonic_partner(x, j);
-                    const bool keep_min = ((tid & j) == 0) == ((gi & k) == 0);
-                    x = keep_min ? (x < y ? x : y) : (x < y ? y : x);
-                }
-            sk[gi] = x;
-        }
-        __syncthreads();
-    }
-    __device__ __forceinline__ void sort_hbm_keys(unsigned long long* sk, int Np) {
-        bitonic_chunks(sk, Np, 2, NT);
-        for (int k = 2 * NT; k <= Np; k <<= 1) {
-            for (int j = k >> 1; j >= NT; j >>= 1) {
-                for (int i = tid; i < Np; i += NT) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const unsigned long long x = sk[i], y = sk[ixj];
-                        const bool up = (i & k) == 0;
-                        if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
-                    }
-                }
-                __syncthreads();
-            }
-            bitonic_chunks(sk, Np, k, k);
-        }
-    }
-
     // ---------------------------------------------------------------- sort + write
     // Orders members then candidates by the enumeration order of reference
     // inc/device_kernels.hpp:31-80 (min(mem) desc, min(can) desc, lvl2adj desc, id desc)
@@ -785,20 +729,18 @@ struct W {
                 sk[i] = key;
             }
             __syncthreads();
-            if (CHUNKED_SORT && Np > SORT_N) sort_hbm_keys(sk, Np);
-            else
-                for (int k = 2; k <= Np; k <<= 1)
-                    for (int j = k >> 1; j > 0; j >>= 1) {
-                        for (int i = tid; i < Np; i += NT) {
-                            int ixj = i ^ j;
-                            if (ixj > i) {
-                                unsigned long long x = sk[i], y = sk[ixj];
-                                bool up = (i & k) == 0;
-                                if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
-                            }
+            for (int k = 2; k <= Np; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < Np; i += NT) {
+                        int ixj = i ^ j;
+                        if (ixj > i) {
+                            unsigned long long x = sk[i], y = sk[ixj];
+                            bool up = (i & k) == 0;
+                            if ((x > y) == up) { sk[i] = y; sk[ixj] = x; }
                         }
-                        __syncthreads();
                     }
+                    __syncthreads();
+                }
             const unsigned long long idmask = bi == 0 ? 0ull : ((1ull << bi) - 1);
             for (int i = tid; i < N; i += NT) {
                 int x = (int)(idx_max - (uint32_t)(sk[i] & idmask));
