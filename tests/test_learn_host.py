@@ -1,4 +1,4 @@
-"""Host side of the learned schedule and of the benchmark driver (CPU only).
+"""Host side of the learned schedule, the autotuner and the benchmark driver (CPU only).
 
 The training-sample builders of cudqc_amd/learn.py are checked on a hand-built task tree
 whose subtree costs are worked out by hand below; the benchmark refuses more than one GPU.
@@ -64,3 +64,13 @@ def test_bench_refuses_several_gpus():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_tuner_applies_knobs_to_a_copy(dqc):
+    from cudqc_amd import tune
+    base = dqc.Config()
+    base.host_addfail_check, base.max_depth, base.spill_tasks = 1, 3, 12345
+    cfg = tune.apply(base, {"pool_per_wg": 16, "split_branches": 256, "split_chunk": None, "resident_steps": 4}, 2048)
+    assert (cfg.pool_tasks, cfg.split_branches, cfg.split_chunk, cfg.resident_steps) == (16 * 2048, 256, 0, 4)
+    assert (cfg.host_addfail_check, cfg.max_depth, cfg.spill_tasks) == (1, 3, 12345)
+    assert (base.pool_tasks, base.split_branches, base.resident_steps) == (0, 0, 0)
