@@ -38,9 +38,13 @@ def apply(base, knobs, workgroups):
 
 
 def measure(graph, gammas, cfg, sched, model=None):
-    """(wall seconds of the expansion, counts, workgroups) of one complete run."""
+    """(wall seconds of the expansion, counts, workgroups) of one complete run. `graph` is
+    a path, or a list of paths enumerated together as one batch."""
     with dq.Engine(cfg) as eng:
-        eng.prepare(graph, *gammas)
+        if isinstance(graph, (list, tuple)):
+            eng.prepare_batch(list(graph), *gammas)
+        else:
+            eng.prepare(graph, *gammas)
         if sched == dq.SCHED_LEARNED:
             eng.set_cost_model(model)
         t = time.perf_counter()
@@ -52,9 +56,9 @@ def measure(graph, gammas, cfg, sched, model=None):
 
 def tune(graph, out_gamma, in_gamma, min_size, max_depth=0, sched=dq.SCHED_SIZE, base=None, grid=GRID,
          model=None, log=None):
-    """Returns (best Config, best knobs, table of (knobs, wall seconds)). `base` supplies
-    every field the grid does not touch (default: the library defaults, 4-rank add-failure
-    checks as in dqc_open(NULL))."""
+    """Returns (best Config, best knobs, table of (knobs, wall seconds)) for one graph or a
+    batch of graphs (a list of paths). `base` supplies every field the grid does not touch
+    (default: the library defaults, 4-rank add-failure checks as in dqc_open(NULL))."""
     if base is None:
         base = dq.Config()
         base.host_addfail_check = 1
