@@ -1439,6 +1439,15 @@ __global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
     }
 }
 
+// Steps a task or piece runs (a piece: its range; a task: all but the covered prefix).
+__host__ __device__ inline int task_steps(const TaskDesc& d) {
+    return (d.flags & TF_PIECE) ? d.iters : d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
+}
+// Least steps of a task that SIZE / LEARNED order a batch for: the split threshold, or
+// 1024 when splitting is off. A batch of shorter tasks keeps stack order, which keeps
+// siblings and the tasks of one graph together; ordering it costs locality and gains
+// no balance.
+__host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0 ? split_b : 1024; }
 // Batch order for the SIZE / LEARNED schedules, one workgroup: tasks go into 64 buckets
 // of descending log2 cost (cost: SIZE = steps x size, LEARNED = the model's prediction
 // in cost[], already computed), and in that order the eligible costliest tasks are
@@ -1449,12 +1458,14 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
                                                       int split_c, unsigned long long budget, uint32_t* __restrict__ perm,
                                                       uint8_t* __restrict__ split) {
     __shared__ unsigned hist[64], start[64];
+    __shared__ unsigned wcnt[POP_T / 64][64];
     __shared__ unsigned long long scan[POP_T];
     __shared__ unsigned long long carry;
+    __shared__ int any_long;
     const unsigned long long n = q->ntask, first = q->first;
     const int tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
-    if (tid == 0) carry = 0;
+    if (tid == 0) { carry = 0; any_long = 0; }
     __syncthreads();
     auto bucket = [&](unsigned long long i) {
         const TaskDesc d = sdesc[first + i];
@@ -1469,15 +1480,52 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
         int b = (int)(2.0 * log2(1.0 + fmax(c, 0.0)));
         return 63 - min(max(b, 0), 63);  // bucket 0 = costliest
     };
-    for (unsigned long long i = tid; i < n; i += POP_T) atomicAdd(&hist[bucket(i)], 1u);
+    for (unsigned long long i = tid; i < n; i += POP_T) {
+        atomicAdd(&hist[bucket(i)], 1u);
+        if (task_steps(sdesc[first + i]) >= long_task_steps(split_b)) any_long = 1;
+    }
     __syncthreads();
+    if (!any_long) {  // no task long enough to split or to unbalance the launch: stack order
+        for (unsigned long long i = tid; i < n; i += POP_T) { perm[i] = (uint32_t)i; split[i] = 0; }
+        return;
+    }
     if (tid == 0) {
         unsigned a = 0;
         for (int b = 0; b < 64; ++b) { start[b] = a; a += hist[b]; }
     }
     __syncthreads();
-    for (unsigned long long i = tid; i < n; i += POP_T) perm[atomicAdd(&start[bucket(i)], 1u)] = (uint32_t)i;
-    __syncthreads();
+    // stable placement: within a bucket, tasks keep their stack order (chunks of POP_T in
+    // index order; inside a chunk, each wave ranks its lanes by ballot, and the waves'
+    // shares follow one another)
+    const int wave = tid >> 6, lane = tid & 63;
+    for (unsigned long long c0 = 0; c0 < n; c0 += POP_T) {
+        const unsigned long long i = c0 + tid;
+        const int b = i < n ? bucket(i) : 64;
+        for (int j = tid; j < (POP_T / 64) * 64; j += POP_T) wcnt[j >> 6][j & 63] = 0;
+        __syncthreads();
+        unsigned long long rem = __ballot(b < 64);
+        unsigned rank = 0;
+        while (rem) {
+            const int lb = __shfl(b, __builtin_ctzll(rem));
+            const unsigned long long m = __ballot(b == lb);
+            if (b == lb) rank = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[wave][lb] = (unsigned)__popcll(m);
+            rem &= ~m;
+        }
+        __syncthreads();
+        if (tid < 64) {  // bucket tid: where each wave's share starts
+            unsigned a = start[tid];
+            for (int w = 0; w < POP_T / 64; ++w) {
+                const unsigned c = wcnt[w][tid];
+                wcnt[w][tid] = a;
+                a += c;
+            }
+            start[tid] = a;
+        }
+        __syncthreads();
+        if (b < 64) perm[wcnt[wave][b] + rank] = (uint32_t)i;
+        __syncthreads();
+    }
     // split marks in run order: eligible tasks while the running piece records fit
     for (unsigned long long c0 = 0; c0 < n; c0 += POP_T) {
         const unsigned long long k = c0 + tid;
@@ -2072,6 +2120,12 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
 // costliest tasks with at least split_branches branches for splitting, within a
 // budget of pool_vertices / 16 piece records per batch (pieces copy the task's records
 // and run all of its branches at once, so the budget bounds the stack growth).
+static bool batch_has_long_task(const TaskDesc* b, unsigned long long n, int split_b) {
+    for (unsigned long long i = 0; i < n; ++i)
+        if (task_steps(b[i]) >= long_task_steps(split_b)) return true;
+    return false;
+}
+
 static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* popped) {
     // newest tasks of the main stack, or of the deferred region when the main stack is empty
     const bool from_main = h->st_t > 0;
@@ -2126,6 +2180,8 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
     const TaskDesc* b = from_main ? rd.data() + (m - cnt) : rd.data();
     const unsigned long long v0 = b[0].off, recs = b[cnt - 1].off + (unsigned long long)b[cnt - 1].tot - v0;
     h->batch_pend = (cnt << CNT_SHIFT) | recs;
+    if (sched != DQC_SCHED_FIFO && !batch_has_long_task(b, cnt, h->cfg.split_branches))
+        sched = DQC_SCHED_FIFO;  // nothing to split or to balance: stack order keeps locality
     if (sched == DQC_SCHED_FIFO) {
         refill_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, first, cnt, h->d_bdesc, h->d_bvtx, 0, 0);
     } else {
