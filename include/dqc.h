@@ -132,9 +132,10 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
                                expanded in stack order, never split                   */
 #define DQC_SCHED_SIZE 2    /* each batch ordered by a size estimate, largest first; the
                                costliest tasks (>= split_branches branches) are cut
-                               into pieces                                            */
-#define DQC_SCHED_LEARNED 3 /* as SIZE, with the subtree-cost predictor of
-                               dqc_set_cost_model as the estimate                     */
+                               into pieces; a batch with no task that long keeps stack
+                               order                                                  */
+#define DQC_SCHED_LEARNED 3 /* as SIZE, with the cost model of dqc_set_cost_model
+                               (predicted branches x task size) as the estimate       */
 dqc_status dqc_run(dqc_handle* h, int32_t sched);
 
 /* Queues any task (same record layout as dqc_push_root): nm members first, then ncov
