@@ -9,7 +9,9 @@ so each part of the oracle is pinned by something independent of it:
   * RemoveNonMax contract: a brute-force subset filter;
   * the whole method: exhaustive subset enumeration of maximal quasi-cliques on small
     digraphs (plus the reference's one documented deviation, see
-    test_whole_method_vs_brute_force).
+    test_whole_method_vs_brute_force);
+  * unequal gammas, where exhaustive enumeration is not the reference's answer: soundness
+    of every reported set in its own directions (a swap of the gammas fails it).
 """
 import itertools
 import math
@@ -178,3 +180,57 @@ def test_oracle_reproduces_golden(oracle_lib, manifest, tmp_path):
         assert r == c["maximal"] and st.tasks == c["tasks"] and st.branches == c["branches"]
         if c["expected"]:
             assert open(mx, "rb").read() == open(os.path.join(GOLDEN, c["expected"]), "rb").read()
+
+
+def run_mapped(oracle_lib, tmp_path, name, arcs, go, gi, mn):
+    """Oracle maximal sets in original ids (the condensed ids mapped back)."""
+    n = len(arcs)
+    path = write(tmp_path, name + ".txt", "".join(" ".join(map(str, sorted(a))) + "\n" for a in arcs))
+    mx = str(tmp_path / (name + ".max"))
+    if os.path.exists(mx):
+        os.remove(mx)
+    oracle_lib.run(path, go, gi, mn, None, mx, threads=1)
+    orig = oracle_lib.condense_map(path, go, gi, mn)
+    got = []
+    if os.path.exists(mx):
+        for line in open(mx):
+            xs = [int(x) for x in line.split()]
+            got.append(tuple(sorted(orig[c] for c in xs[1:])))
+    return sorted(got), len(orig)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_unequal_gammas_sound_and_direction_aware(oracle_lib, tmp_path, seed):
+    """gamma_in != gamma_out, where exhaustive enumeration is not the reference's answer
+    (its same-direction LU test can prune a valid set). What the method must still do:
+    every set it reports is a quasi-clique for the two thresholds in their own directions
+    (out-degree inside the set >= ceil(gamma_out (k-1)), in-degree >= ceil(gamma_in (k-1)),
+    k >= min_size), no reported set contains another, and every reported set lies inside a
+    maximal quasi-clique of the brute-force enumeration. A swap of the two gammas, or of
+    the out and in lists, breaks the first check on these asymmetric graphs."""
+    rng = random.Random(500 + seed)
+    nonempty = 0
+    for t in range(120):
+        n = rng.randint(5, 11)
+        p = rng.choice([0.7, 0.8, 0.9, 0.95])
+        go, gi = rng.choice([(0.9, 0.5), (0.5, 0.9), (0.8, 0.6), (1.0, 0.6), (0.6, 1.0)])
+        mn = rng.randint(3, 6)
+        # asymmetric density: a forward-heavy orientation makes in- and out-degrees differ
+        arcs = [set(v for v in range(n) if v != u and rng.random() < (p if v > u else 0.7 * p)) for u in range(n)]
+        out = [sum(1 << v for v in arcs[u]) for u in range(n)]
+        inn = [sum(1 << u for u in range(n) if v in arcs[u]) for v in range(n)]
+        got, k_cond = run_mapped(oracle_lib, tmp_path, f"u{t}", arcs, go, gi, mn)
+        if k_cond < mn:
+            continue  # the documented level-0 deviation (see test_whole_method_vs_brute_force)
+        brute = [set(s) for s in brute_maximal(n, out, inn, go, gi, mn)]
+        for s in got:
+            k = len(s)
+            mask = sum(1 << v for v in s)
+            assert k >= mn, (t, s)
+            for v in s:
+                assert bin(out[v] & mask).count("1") >= md(go, k), (t, s, v, "out")
+                assert bin(inn[v] & mask).count("1") >= md(gi, k), (t, s, v, "in")
+            assert any(set(s) <= b for b in brute), (t, s)
+        assert not any(set(a) < set(b) for a in got for b in got), t
+        nonempty += bool(got)
+    assert nonempty > 20
