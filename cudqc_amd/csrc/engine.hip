@@ -1450,9 +1450,11 @@ __host__ __device__ inline int task_steps(const TaskDesc& d) {
 __host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0 ? split_b : 1024; }
 // Batch order for the SIZE / LEARNED schedules, one workgroup: tasks go into 64 buckets
 // of descending log2 cost (cost: SIZE = steps x size, LEARNED = the model's prediction
-// in cost[], already computed), and in that order the eligible costliest tasks are
-// marked for splitting while their piece records fit `budget`. perm[k] = the batch index
-// of the k-th task to run; split[k] = 1 to split it.
+// x size, the prediction already in pred[]), keeping stack order within a bucket, and in
+// that order the eligible costliest tasks are marked for splitting while their piece
+// records fit `budget`. A batch without a task of long_task_steps steps keeps stack
+// order and splits nothing. perm[k] = the batch index of the k-th task to run; split[k]
+// = 1 to split it.
 __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
                                                       const float* __restrict__ pred, int learned, int split_b,
                                                       int split_c, unsigned long long budget, uint32_t* __restrict__ perm,
