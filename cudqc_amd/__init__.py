@@ -22,7 +22,7 @@ EXPORTS = [
     "dqc_push_task", "dqc_fetch_tasks", "dqc_reset", "dqc_set_task_log", "dqc_fetch_task_log",
     "dqc_task_features", "dqc_mlp_forward", "dqc_mlp_gradient", "dqc_mlp_train", "dqc_set_cost_model",
     "dqc_upload_graph_batch", "dqc_push_graph_root", "dqc_fetch_batch_cliques", "dqc_prepare_batch",
-    "dqc_search_batch",
+    "dqc_search_batch", "dqc_min_degrees", "dqc_search_graph", "dqc_prepare_graph", "dqc_save_graph",
 ]
 
 MLP_IN, MLP_HIDDEN = 7, 64
@@ -60,6 +60,7 @@ class Stats(ctypes.Structure):
         ("n_condensed", ctypes.c_int32), ("workgroups", ctypes.c_int32),
         ("truncated_children", ctypes.c_int64), ("stack_pushbacks", ctypes.c_int64),
         ("peak_stack_tasks", ctypes.c_int64), ("peak_stack_records", ctypes.c_int64),
+        ("preprocess_ms", ctypes.c_double), ("rmnonmax_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -68,6 +69,10 @@ class Stats(ctypes.Structure):
 
 class DQCError(RuntimeError):
     pass
+
+
+# dqc_set_sink: receives each maximal set
+SET_SINK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32)
 
 
 _lib = None
@@ -152,6 +157,20 @@ def load():
                                      ctypes.POINTER(Config), PI64, ctypes.POINTER(Stats)]
     lib.dqc_fetch_batch_cliques.restype = ctypes.c_int32
     lib.dqc_fetch_batch_cliques.argtypes = [ctypes.c_void_p, PI64, PI64, P32, P32, P32]
+    lib.dqc_min_degrees.restype = ctypes.c_int32
+    lib.dqc_min_degrees.argtypes = [ctypes.c_double, ctypes.c_int32, P32]
+    csr = [P64, P32, P64, P32, P64, P32]
+    lib.dqc_search_graph.restype = ctypes.c_int32
+    lib.dqc_search_graph.argtypes = [ctypes.c_int32] + csr + [P32, P32, ctypes.c_int32, ctypes.c_double,
+                                                              ctypes.c_double, ctypes.c_int32, ctypes.c_char_p,
+                                                              ctypes.c_char_p, SET_SINK, ctypes.c_void_p,
+                                                              ctypes.POINTER(Config), PI64, ctypes.POINTER(Stats), P32]
+    lib.dqc_prepare_graph.restype = ctypes.c_int32
+    lib.dqc_prepare_graph.argtypes = [ctypes.c_void_p, ctypes.c_int32] + csr + [P32, P32, ctypes.c_int32,
+                                                                                ctypes.c_double, ctypes.c_double,
+                                                                                ctypes.c_int32, P32, P32]
+    lib.dqc_save_graph.restype = ctypes.c_int32
+    lib.dqc_save_graph.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     _lib = lib
     return lib
 
@@ -278,6 +297,25 @@ class Engine:
         k = ctypes.c_int32()
         self._check(self.lib.dqc_prepare_file(self.h, _b(graph_path), out_gamma, in_gamma, int(min_size),
                                               ctypes.byref(k)), "dqc_prepare_file")
+        return k.value
+
+    def prepare_graph(self, g, out_gamma, in_gamma, min_size, md_out=None, md_in=None):
+        """dqc_prepare_graph on a graph given as a dict of adjacency lists (out, in, and
+        optionally two); returns the condensed vertex count."""
+        n = len(g["out"])
+        oo, on = _csr_arrays(g["out"])
+        io, inn = _csr_arrays(g["in"])
+        to, tn = _csr_arrays(g["two"]) if g.get("two") is not None else (None, None)
+        ml = 0
+        mo = mi = None
+        if md_out is not None:
+            ml = min(len(md_out), len(md_in))
+            mo = (ctypes.c_int32 * ml)(*md_out[:ml])
+            mi = (ctypes.c_int32 * ml)(*md_in[:ml])
+        k = ctypes.c_int32()
+        self._check(self.lib.dqc_prepare_graph(self.h, n, oo, on, io, inn, to, tn, mo, mi, ml, float(out_gamma),
+                                               float(in_gamma), int(min_size), ctypes.byref(k), None),
+                    "dqc_prepare_graph")
         return k.value
 
     def step(self, launches=1, sched=SCHED_DEFAULT):
@@ -407,6 +445,69 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+
+def min_degrees(gamma, n):
+    """h_calculate_minimum_degrees: md[0..n] (reference src/host_functions.cu:9-15)."""
+    arr = (ctypes.c_int32 * (n + 1))()
+    rc = load().dqc_min_degrees(float(gamma), int(n), arr)
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_min_degrees failed: {STATUS.get(rc, rc)}")
+    return list(arr)
+
+
+def _csr_arrays(lists):
+    """Python adjacency lists -> (ctypes uint64 offsets, ctypes int32 ids)."""
+    off = [0]
+    for l in lists:
+        off.append(off[-1] + len(l))
+    o = (ctypes.c_uint64 * len(off))(*off)
+    flat = [x for l in lists for x in l]
+    ids = (ctypes.c_int32 * max(len(flat), 1))(*flat)
+    return o, ids
+
+
+def search_graph(out_lists, in_lists, two_lists, md_out, md_in, out_gamma, in_gamma, min_size, raw_path=None,
+                 max_path=None, config=None, want_orig=False):
+    """In-memory h_search (dqc_search_graph): adjacency lists of an uncondensed graph
+    (two_lists None: built by the library), the caller's minimum-degree tables (None:
+    the closed form). Returns (maximal sets in DQC-R order, stats[, orig_of])."""
+    lib = load()
+    n = len(out_lists)
+    oo, on = _csr_arrays(out_lists)
+    io, inn = _csr_arrays(in_lists)
+    to, tn = _csr_arrays(two_lists) if two_lists is not None else (None, None)
+    ml = 0
+    mo = mi = None
+    if md_out is not None:
+        ml = min(len(md_out), len(md_in))
+        mo = (ctypes.c_int32 * ml)(*md_out[:ml])
+        mi = (ctypes.c_int32 * ml)(*md_in[:ml])
+    sets = []
+
+    def sink(_ctx, ids, length):
+        sets.append(tuple(ids[i] for i in range(length)))
+
+    cb = SET_SINK(sink)
+    cnt = ctypes.c_int64()
+    st = Stats()
+    orig = (ctypes.c_int32 * max(n, 1))()
+    rc = lib.dqc_search_graph(n, oo, on, io, inn, to, tn, mo, mi, ml, float(out_gamma), float(in_gamma), int(min_size),
+                              _b(raw_path), _b(max_path), cb, None, ctypes.byref(config) if config else None,
+                              ctypes.byref(cnt), ctypes.byref(st), orig)
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_search_graph failed: {STATUS.get(rc, rc)}")
+    assert cnt.value == len(sets)
+    if want_orig:
+        return sets, st, [orig[i] for i in range(st.n_condensed)]
+    return sets, st
+
+
+def save_graph(graph_path, bin_path, with_twohop=True):
+    """Binary graph cache of a text (or binary) graph file (dqc_save_graph)."""
+    rc = load().dqc_save_graph(_b(graph_path), _b(bin_path), int(bool(with_twohop)))
+    if rc != DQC_OK:
+        raise DQCError(f"dqc_save_graph failed: {STATUS.get(rc, rc)}")
 
 
 def remove_non_max(in_path, out_path):

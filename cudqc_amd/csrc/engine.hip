@@ -104,7 +104,11 @@ struct QCtr {
     unsigned long long wg_limit;   // workgroups the stack has room for (device-resident steps)
     unsigned long long def_alloc;  // deferred region, growing down from the top of the stack
                                    // arrays: (pieces << 34) | records
-    unsigned long long pad[2];
+    unsigned long long used;       // main + deferred: (tasks << 34) | records. Every push adds
+                                   // to it first and tests the capacity on the value it returns,
+                                   // so two regions can never be filled past each other
+    unsigned long long split_budget;  // device-resident steps: piece records the order kernel
+                                      // may mark for splitting in the current batch
 };
 // Counters shared by every launch of a handle.
 struct GCtr {
@@ -117,7 +121,7 @@ struct GCtr {
     unsigned long long log_n;      // task-log records requested
     unsigned long long steps;      // device-resident steps that expanded a batch
     unsigned long long truncated;  // children not pushed: deeper than max_depth
-    unsigned long long spill_state[4];  // first failed stack write: top, request, launch state, pending
+    unsigned long long spill_state[4];  // first failed stack write: use, request, launch state, pending
     unsigned long long pushbacks;  // tasks whose rest went back to the stack for lack of room
 };
 
@@ -692,16 +696,7 @@ struct W {
         const int bi = bitw(idx_max), bl = bitw(l2x), bc = bitw(mcx), bm = bitw(mmx);
         const int total = 1 + bm + bc + bl + bi;
         unsigned long long a = 0;
-        if (tid == 0) {
-            const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)N;
-            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(&P->q->sp_alloc, inc);
-            const unsigned long long dq = __atomic_load_n(&P->q->def_alloc, __ATOMIC_RELAXED);
-            if (a != ~0ull && ((a >> CNT_SHIFT) + (dq >> CNT_SHIFT) >= P->sp_cap_t ||
-                               (a & SIZE_MASK) + (dq & SIZE_MASK) + N > P->sp_cap_v)) {
-                note_spill(a, (unsigned long long)N);
-                a = ~0ull;
-            }
-        }
+        if (tid == 0) a = reserve(false, (unsigned long long)N);
         a = bcast0(a);
         if (a == ~0ull) return -1;
         const unsigned long long t = a >> CNT_SHIFT, o = a & SIZE_MASK;
@@ -785,8 +780,25 @@ struct W {
         return (long long)t;
     }
 
-    // Flags a stack overflow; the first one records the stack top, the request (bit 63:
-    // a piece), this workgroup's launch state and the pending batch share.
+    // Thread 0: room for one push of `recs` records, onto the main stack or (deferred)
+    // the deferred region. The capacity is tested on the combined main + deferred use
+    // that the one atomic on q->used returns, so concurrent pushes to the two regions
+    // cannot both pass on a stale view of the other; only then is the slot taken in its
+    // region. Returns the region's previous (tasks << 34) | records, or ~0ull when full
+    // or after an error.
+    __device__ __forceinline__ unsigned long long reserve(bool deferred, unsigned long long recs) {
+        if (__atomic_load_n(&P->g->error, __ATOMIC_RELAXED)) return ~0ull;
+        const unsigned long long inc = (1ull << CNT_SHIFT) | recs;
+        const unsigned long long u = atomicAdd(&P->q->used, inc);
+        if ((u >> CNT_SHIFT) + 1 > P->sp_cap_t || (u & SIZE_MASK) + recs > P->sp_cap_v) {
+            note_spill(u, recs | (deferred ? (1ull << 63) : 0ull));
+            return ~0ull;
+        }
+        return atomicAdd(deferred ? &P->q->def_alloc : &P->q->sp_alloc, inc);
+    }
+
+    // Flags a stack overflow; the first one records the stack use, the request (bit 63:
+    // a deferred piece), this workgroup's launch state and the pending batch share.
     __device__ __forceinline__ void note_spill(unsigned long long top, unsigned long long req) {
         if (atomicOr(&P->g->error, (unsigned)ERR_SPILL) & ERR_SPILL) return;
         P->g->spill_state[0] = top;
@@ -807,8 +819,7 @@ struct W {
     __device__ __forceinline__ bool stack_low() {
         unsigned long long low = 0;
         if (tid == 0) {
-            const unsigned long long a = __atomic_load_n(&P->q->sp_alloc, __ATOMIC_RELAXED) +
-                                         __atomic_load_n(&P->q->def_alloc, __ATOMIC_RELAXED);
+            const unsigned long long a = __atomic_load_n(&P->q->used, __ATOMIC_RELAXED);
             const unsigned long long pend = __atomic_load_n(&P->q->pend, __ATOMIC_RELAXED);
             const unsigned long long mt = P->maxtot == ~0ull ? P->q->maxtot : P->maxtot;
             const unsigned long long wg = P->maxtot == ~0ull ? P->q->wg_limit : gridDim.x;
@@ -829,18 +840,7 @@ struct W {
     __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters,
                                                 bool deferred = false) {
         unsigned long long a = 0;
-        if (tid == 0) {
-            const unsigned long long inc = (1ull << CNT_SHIFT) | (unsigned long long)tot;
-            unsigned long long* ctr = deferred ? &P->q->def_alloc : &P->q->sp_alloc;
-            unsigned long long* other = deferred ? &P->q->sp_alloc : &P->q->def_alloc;
-            a = __atomic_load_n(&P->g->error, __ATOMIC_RELAXED) ? ~0ull : atomicAdd(ctr, inc);
-            const unsigned long long b = __atomic_load_n(other, __ATOMIC_RELAXED);
-            if (a != ~0ull && ((a >> CNT_SHIFT) + (b >> CNT_SHIFT) >= P->sp_cap_t ||
-                               (a & SIZE_MASK) + (b & SIZE_MASK) + tot > P->sp_cap_v)) {
-                note_spill(a, (unsigned long long)tot | (1ull << 63));
-                a = ~0ull;
-            }
-        }
+        if (tid == 0) a = reserve(deferred, (unsigned long long)tot);
         a = bcast0(a);
         if (a == ~0ull) return;
         // the deferred region fills the arrays from the top down: its newest piece has the
@@ -1207,13 +1207,17 @@ struct W {
         const bool remove_first = piece && (d.flags & TF_REMOVE_FIRST);
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
-            if (stack_low()) {  // backpressure: the rest of the task becomes a deferred piece
+            // The split pass writes only at chunk starts, so it checks the stack only there:
+            // a deferred piece then takes the place of that chunk's piece and no step is
+            // covered twice (a check inside a chunk would defer steps its piece already holds).
+            const bool chunk_start = (j - ncov) % chunk == 0;
+            if ((!split || chunk_start) && stack_low()) {
+                // backpressure: the rest of the task becomes a deferred piece
                 write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, true);
                 if (tid == 0) atomicAdd(&P->g->pushbacks, 1ull);
                 break;
             }
-            if (split && (j - ncov) % chunk == 0)
-                write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
+            if (split && chunk_start) write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
             if (j != ncov || remove_first) {
                 tick(0);
                 int r = remove_one(tot, nc, nm);
@@ -1423,6 +1427,14 @@ __global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
         q.maxtot = mtot;
         q.wg_limit = wgl;
         q.pend = 0;
+        // piece-record budget of the split marks, clamped by the free stack records the way
+        // the host pop clamps it (use counted before the pop)
+        {
+            const unsigned long long used_v = st_v + df_v;
+            unsigned long long budget = pp.pool_vertices / 16;
+            if (used_v + budget > pp.spill_vertices) budget = pp.spill_vertices > used_v ? (pp.spill_vertices - used_v) / 2 : 0;
+            q.split_budget = budget;
+        }
         if (cnt > 0) {
             // first / v0: the batch's lowest descriptor index and record; both regions keep a
             // batch contiguous (gather_kernel copies [v0, last.off + last.tot))
@@ -1436,6 +1448,7 @@ __global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
             else q.def_alloc = ((df_t - cnt) << CNT_SHIFT) | (df_v - recs);
             q.pend = (cnt << CNT_SHIFT) | recs;
         }
+        q.used = q.sp_alloc + q.def_alloc;  // field-wise sum: neither field carries
     }
 }
 
@@ -1451,20 +1464,21 @@ __host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0
 // Batch order for the SIZE / LEARNED schedules, one workgroup: tasks go into 64 buckets
 // of descending log2 cost (cost: SIZE = steps x size, LEARNED = the model's prediction
 // x size, the prediction already in pred[]), keeping stack order within a bucket, and in
-// that order the eligible costliest tasks are marked for splitting while their piece
-// records fit `budget`. A batch without a task of long_task_steps steps keeps stack
-// order and splits nothing. perm[k] = the batch index of the k-th task to run; split[k]
-// = 1 to split it.
+// that order the eligible tasks are marked for splitting while the running total of
+// their piece records fits q->split_budget (a prefix rule: the first task that does not
+// fit ends the marking, as in the host pop). A batch without a task of long_task_steps
+// steps keeps stack order and splits nothing. perm[k] = the batch index of the k-th
+// task to run; split[k] = 1 to split it.
 __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
                                                       const float* __restrict__ pred, int learned, int split_b,
-                                                      int split_c, unsigned long long budget, uint32_t* __restrict__ perm,
+                                                      int split_c, uint32_t* __restrict__ perm,
                                                       uint8_t* __restrict__ split) {
     __shared__ unsigned hist[64], start[64];
     __shared__ unsigned wcnt[POP_T / 64][64];
     __shared__ unsigned long long scan[POP_T];
     __shared__ unsigned long long carry;
     __shared__ int any_long;
-    const unsigned long long n = q->ntask, first = q->first;
+    const unsigned long long n = q->ntask, first = q->first, budget = q->split_budget;
     const int tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
     if (tid == 0) { carry = 0; any_long = 0; }
@@ -2119,9 +2133,12 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
 //
 // FIFO keeps stack order. SIZE and LEARNED order the batch by an estimate of each
 // task's cost, largest first, so that the longest tasks start first, and mark the
-// costliest tasks with at least split_branches branches for splitting, within a
-// budget of pool_vertices / 16 piece records per batch (pieces copy the task's records
-// and run all of its branches at once, so the budget bounds the stack growth).
+// costliest tasks with at least split_branches branches for splitting, in that order
+// while the running total of their piece records fits a budget of pool_vertices / 16
+// per batch (half the free stack records when less is free; pieces copy the task's
+// records and run all of its branches at once, so the budget bounds the stack growth).
+// The first eligible task that does not fit ends the marking; the device-resident
+// order kernel follows the same prefix rule with the same budget.
 static bool batch_has_long_task(const TaskDesc* b, unsigned long long n, int split_b) {
     for (unsigned long long i = 0; i < n; ++i)
         if (task_steps(b[i]) >= long_task_steps(split_b)) return true;
@@ -2208,6 +2225,8 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
                 if (need <= budget) {
                     budget -= need;
                     d.flags |= TF_SPLIT;
+                } else {
+                    budget = 0;  // prefix rule: no later task is split in this batch
                 }
             }
             h->hbatch[i] = d;
@@ -2326,6 +2345,7 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     q.root_first = ~0ull;
     q.pend = P.split_s ? 0 : h->batch_pend;
     q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
+    q.used = ((h->st_t + h->def_t) << CNT_SHIFT) | (h->st_v + h->def_v);
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
     unsigned long long gmax = (unsigned long long)h->launch_wg;
     if (!P.split_s) gmax = std::min(gmax, h->batch_grid);
@@ -2347,7 +2367,7 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     fold_stats(h, g);
     if (g.error) {
         char st[200];
-        std::snprintf(st, sizeof st, " [top %llu/%llu, request %llu, state %llx, pending %llu/%llu]",
+        std::snprintf(st, sizeof st, " [used %llu/%llu, request %llu, state %llx, pending %llu/%llu]",
                       g.spill_state[0] >> CNT_SHIFT, g.spill_state[0] & SIZE_MASK, g.spill_state[1], g.spill_state[2],
                       g.spill_state[3] >> CNT_SHIFT, g.spill_state[3] & SIZE_MASK);
         h->err = (g.error & ERR_SPILL) ? std::string("task stack full (raise spill_tasks / spill_vertices)") + st
@@ -2438,8 +2458,7 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
             desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc, 0, h->d_cost, h->d_q);
         }
         order_kernel<<<1, POP_T, 0, h->stream>>>(h->d_sdesc, h->d_q, h->d_cost, sched == DQC_SCHED_LEARNED,
-                                                 h->cfg.split_branches, h->cfg.split_chunk, h->cfg.pool_vertices / 16,
-                                                 h->d_perm, h->d_mark);
+                                                 h->cfg.split_branches, h->cfg.split_chunk, h->d_perm, h->d_mark);
         perm = h->d_perm;
         mark = h->d_mark;
     }
@@ -2461,6 +2480,7 @@ static dqc_status resident_run(dqc_handle* h, int32_t sched, int64_t max_steps, 
     QCtr q{};
     q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
     q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
+    q.used = ((h->st_t + h->def_t) << CNT_SHIFT) | (h->st_v + h->def_v);
     q.root_first = ~0ull;
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
     hipGraphExec_t& ex = h->gexec[sched];

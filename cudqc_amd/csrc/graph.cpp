@@ -88,6 +88,141 @@ int load_text(const std::string& path, Graph& g, std::string& err) {
 }
 
 // ---------------------------------------------------------------------------
+// Binary graph cache (no reference counterpart: the reference parses the text file on
+// every run, src/common.cpp:28-57).
+namespace {
+constexpr char BIN_MAGIC[8] = {'D', 'Q', 'C', 'G', 'R', 'P', 'H', '1'};
+
+// Word-wise 64-bit hash over a byte stream (multiply-xorshift per 8-byte word; the
+// tail is zero-padded). Detects truncation and corruption of a cache file.
+struct StreamHash {
+    uint64_t h = 0x9E3779B97F4A7C15ull, pending = 0;
+    int npend = 0;
+    void mix(uint64_t w) {
+        h ^= w + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h *= 0xFF51AFD7ED558CCDull;
+        h ^= h >> 33;
+    }
+    void add(const void* p, size_t n) {
+        const unsigned char* b = (const unsigned char*)p;
+        while (n && npend) {
+            pending |= (uint64_t)*b++ << (8 * npend);
+            --n;
+            if (++npend == 8) { mix(pending); pending = 0; npend = 0; }
+        }
+        for (; n >= 8; n -= 8, b += 8) {
+            uint64_t w;
+            std::memcpy(&w, b, 8);
+            mix(w);
+        }
+        while (n--) { pending |= (uint64_t)*b++ << (8 * npend); ++npend; }
+    }
+    uint64_t final() {
+        if (npend) mix(pending);
+        mix(0x5A17ull);
+        return h;
+    }
+};
+
+bool put(FILE* f, StreamHash& hs, const void* p, size_t n) {
+    hs.add(p, n);
+    return n == 0 || std::fwrite(p, 1, n, f) == n;
+}
+bool get(FILE* f, StreamHash& hs, void* p, size_t n) {
+    if (n && std::fread(p, 1, n, f) != n) return false;
+    hs.add(p, n);
+    return true;
+}
+bool put_csr(FILE* f, StreamHash& hs, const Csr& c, int32_t n) {
+    return put(f, hs, c.off.data(), sizeof(uint64_t) * ((size_t)n + 1)) &&
+           put(f, hs, c.nbr.data(), sizeof(int32_t) * c.nbr.size());
+}
+// reads one CSR whose entry count is given by its own last offset (checked against max_len)
+bool get_csr(FILE* f, StreamHash& hs, Csr& c, int32_t n, uint64_t max_len, std::string& err) {
+    c.off.resize((size_t)n + 1);
+    if (!get(f, hs, c.off.data(), sizeof(uint64_t) * ((size_t)n + 1))) { err = "truncated binary graph"; return false; }
+    if (c.off[0] != 0) { err = "binary graph: offsets do not start at 0"; return false; }
+    for (int32_t v = 0; v < n; ++v)
+        if (c.off[v + 1] < c.off[v]) { err = "binary graph: decreasing offsets"; return false; }
+    if (c.off[n] > max_len) { err = "binary graph: list longer than the file allows"; return false; }
+    c.nbr.resize(c.off[n]);
+    if (!get(f, hs, c.nbr.data(), sizeof(int32_t) * c.nbr.size())) { err = "truncated binary graph"; return false; }
+    for (int32_t x : c.nbr)
+        if (x < 0 || x >= n) { err = "binary graph: id outside [0, n)"; return false; }
+    return true;
+}
+}  // namespace
+
+bool save_binary(const std::string& path, const Graph& g, std::string& err) {
+    const bool two = g.two.off.size() == (size_t)g.n + 1 && g.two.off[g.n] > 0;
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { err = "cannot write " + path; return false; }
+    StreamHash hs;
+    const int32_t flags = two ? 1 : 0;
+    const int64_t m = g.m;
+    bool ok = put(f, hs, BIN_MAGIC, 8) && put(f, hs, &g.n, 4) && put(f, hs, &flags, 4) && put(f, hs, &m, 8) &&
+              put_csr(f, hs, g.out, g.n) && put_csr(f, hs, g.in, g.n) && (!two || put_csr(f, hs, g.two, g.n));
+    const uint64_t sum = hs.final();
+    ok = ok && std::fwrite(&sum, 8, 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) err = "short write on " + path;
+    return ok;
+}
+
+bool is_binary_graph(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char m[8] = {0};
+    const bool r = std::fread(m, 1, 8, f) == 8 && std::memcmp(m, BIN_MAGIC, 8) == 0;
+    std::fclose(f);
+    return r;
+}
+
+bool load_binary(const std::string& path, Graph& g, std::string& err, bool* has_twohop) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) { err = "cannot open graph file " + path; return false; }
+    std::fseek(f, 0, SEEK_END);
+    const long long fsz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    StreamHash hs;
+    char magic[8];
+    int32_t n = 0, flags = 0;
+    int64_t m = 0;
+    bool ok = get(f, hs, magic, 8) && std::memcmp(magic, BIN_MAGIC, 8) == 0 && get(f, hs, &n, 4) &&
+              get(f, hs, &flags, 4) && get(f, hs, &m, 8) && n >= 0 && m >= 0 && (flags & ~1) == 0;
+    if (!ok) { std::fclose(f); err = "not a binary graph file: " + path; return false; }
+    // every list is bounded by the file size (an offset table cannot ask for more)
+    const uint64_t max_len = fsz > 0 ? (uint64_t)fsz / 4 : 0;
+    Graph r;
+    r.n = n;
+    r.m = m;
+    ok = get_csr(f, hs, r.out, n, max_len, err) && get_csr(f, hs, r.in, n, max_len, err);
+    if (ok && (r.out.off[n] != (uint64_t)m || r.in.off[n] != (uint64_t)m)) { err = "binary graph: arc counts differ"; ok = false; }
+    if (ok && (flags & 1)) ok = get_csr(f, hs, r.two, n, max_len, err);
+    uint64_t sum = 0;
+    if (ok && (std::fread(&sum, 8, 1, f) != 1 || sum != hs.final())) { err = "binary graph: checksum mismatch"; ok = false; }
+    std::fclose(f);
+    if (!ok) return false;
+    if (!(flags & 1)) { r.two.off.assign((size_t)n + 1, 0); r.two.nbr.clear(); }
+    if (has_twohop) *has_twohop = (flags & 1) != 0;
+    g = std::move(r);
+    return true;
+}
+
+int load_graph_file(const std::string& path, Graph& g, std::string& err, int threads) {
+    if (is_binary_graph(path)) {
+        bool two = false;
+        if (!load_binary(path, g, err, &two)) return 2;
+        if (!two) build_twohop(g, threads);
+        return 0;
+    }
+    const int rc = load_text(path, g, err);
+    if (rc) return rc;
+    build_twohop(g, threads);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // reference src/common.cpp:111-509 (GenLevel2NBs). For a vertex v with both out-
 // and in-arcs: B = mutual neighbours, O = out-only, I = in-only. The one-direction
 // sets are pruned to a greatest fixed point (w stays in I while an arc reaches it

@@ -37,6 +37,23 @@ int load_text(const std::string& path, Graph& g, std::string& err);
 // Directed two-hop lists (reference src/common.cpp:111-509 semantics).
 void build_twohop(Graph& g, int threads);
 
+// Binary graph file, a cache of what the text loader (and optionally the two-hop
+// builder) produces, so that a large graph is parsed and its two-hop lists built once.
+// Layout (little-endian): the 8 bytes "DQCGRPH1", int32 n, int32 flags (bit 0: two-hop
+// lists present), int64 m, then out offsets (n + 1 uint64), out ids (m int32), in
+// offsets, in ids, and, when flagged, two-hop offsets and ids; last, a 64-bit checksum
+// of every byte before it. Loading checks the magic, the sizes, that offsets start at 0
+// and never decrease, that ids lie in [0, n), and the checksum.
+bool save_binary(const std::string& path, const Graph& g, std::string& err);
+// Returns false with err set on any failure; g.two stays empty when the file holds no
+// two-hop lists (two.off all zero).
+bool load_binary(const std::string& path, Graph& g, std::string& err, bool* has_twohop = nullptr);
+// True when the file starts with the binary magic.
+bool is_binary_graph(const std::string& path);
+// Text or binary graph, with its two-hop lists built when the file does not hold them.
+// Returns 0, 1 on I/O failure, 2 on malformed content.
+int load_graph_file(const std::string& path, Graph& g, std::string& err, int threads);
+
 // md[0] = 0, md[k] = ceil(gamma (k-1)) in IEEE double (reference
 // src/host_functions.cu:9-15), for k < len.
 std::vector<int32_t> min_degree_table(double gamma, int len);

@@ -72,7 +72,13 @@ typedef struct {
                                   the task stack was nearly full                      */
     int64_t peak_stack_tasks;  /* largest task-stack use seen between launches        */
     int64_t peak_stack_records;
+    double preprocess_ms;      /* search entry points: graph load (or adoption), two-hop
+                                  lists, level 0 + condense and upload                 */
+    double rmnonmax_ms;        /* search entry points: the maximal-set filter         */
 } dqc_stats;
+
+/* Receives each maximal set (ascending ids, sets in the DQC-R file's order). */
+typedef void (*dqc_set_sink)(void* ctx, const int32_t* ids, int32_t len);
 
 /* One record per expanded task (or piece of a split task): the task's id, its parent's
  * id (UINT32_MAX for tasks pushed by the host), depth, members, size, the min / max out-
@@ -216,9 +222,52 @@ dqc_status dqc_search_file(const char* graph_path, double out_gamma, double in_g
                            int32_t min_size, const char* raw_path, const char* max_path,
                            const dqc_config* cfg, int64_t* max_cliques, dqc_stats* st);
 
-/* Host preprocessing, no GPU needed. dqc_load_graph: text loader (+ two-hop lists when
- * with_twohop) with the semantics of CPU_Graph::CPU_Graph / GenLevel2NBs (reference
- * src/common.cpp:15-509); arrays are malloc'd, release them with dqc_free. */
+/* h_calculate_minimum_degrees (reference inc/host_functions.hpp:7,
+ * src/host_functions.cu:9-15): md[0] = 0, md[k] = ceil(gamma (k - 1)) in IEEE double for
+ * k = 1..n (n + 1 entries). No GPU needed. */
+dqc_status dqc_min_degrees(double gamma, int32_t n, int32_t* md);
+
+/* In-memory form of the reference's h_search (inc/host_functions.hpp:8-10) over an
+ * uncondensed CPU_Graph (inc/common.hpp:76-94): out / in CSR with 64-bit offsets, each
+ * list strictly ascending without self loops, in-lists the transpose of the out-lists
+ * (checked; DQC_E_ARG otherwise); two-hop lists as GenLevel2NBs builds them, or NULL
+ * two_off / two_nbr to have them built here. md_out / md_in are the caller's minimum-
+ * degree tables (h_calculate_minimum_degrees output, md_len entries, used verbatim;
+ * indices beyond md_len continue the closed form with the same gamma; md_len 0 with
+ * NULL tables: the closed form throughout). Level 0 + condense, the GPU expansion,
+ * RemoveNonMax. Outputs: raw_path (DQC-T_ format, the temp_results stream of
+ * h_search) and max_path (DQC-R_ format, not created when empty), each optional; sink,
+ * when given, receives every maximal set; orig_of (optional, n entries) gets the input
+ * id of each condensed id, which the result files use (the reference discards this
+ * map, src/host_functions.cu:919). Unlike h_search, the caller's arrays are not
+ * modified. */
+dqc_status dqc_search_graph(int32_t n, const uint64_t* out_off, const int32_t* out_nbr,
+                            const uint64_t* in_off, const int32_t* in_nbr,
+                            const uint64_t* two_off, const int32_t* two_nbr,
+                            const int32_t* md_out, const int32_t* md_in, int32_t md_len,
+                            double out_gamma, double in_gamma, int32_t min_size,
+                            const char* raw_path, const char* max_path,
+                            dqc_set_sink sink, void* sink_ctx, const dqc_config* cfg,
+                            int64_t* max_cliques, dqc_stats* st, int32_t* orig_of);
+/* The preprocessing half of dqc_search_graph into an open handle (as dqc_prepare_file):
+ * checks, level 0 + condense, upload, level-0 task queued; ready for dqc_step / dqc_run. */
+dqc_status dqc_prepare_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, const int32_t* out_nbr,
+                             const uint64_t* in_off, const int32_t* in_nbr,
+                             const uint64_t* two_off, const int32_t* two_nbr,
+                             const int32_t* md_out, const int32_t* md_in, int32_t md_len,
+                             double out_gamma, double in_gamma, int32_t min_size,
+                             int32_t* n_condensed, int32_t* orig_of);
+
+/* Binary graph cache (replaces re-parsing the text file, reference src/common.cpp:28-57):
+ * loads graph_path (text or binary) and writes bin_path in the binary format, with the
+ * two-hop lists when with_twohop (built if the input lacks them). Every entry point
+ * that takes a graph path accepts either format (detected by its magic bytes); a binary
+ * file with two-hop lists skips building them. No GPU needed. */
+dqc_status dqc_save_graph(const char* graph_path, const char* bin_path, int32_t with_twohop);
+
+/* Host preprocessing, no GPU needed. dqc_load_graph: text or binary loader (+ two-hop
+ * lists when with_twohop) with the semantics of CPU_Graph::CPU_Graph / GenLevel2NBs
+ * (reference src/common.cpp:15-509); arrays are malloc'd, release them with dqc_free. */
 dqc_status dqc_load_graph(const char* graph_path, int32_t with_twohop, int32_t* n, int64_t* m,
                           uint64_t** out_off, int32_t** out_nbr, uint64_t** in_off, int32_t** in_nbr,
                           uint64_t** two_off, int32_t** two_nbr);

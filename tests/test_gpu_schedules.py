@@ -131,18 +131,36 @@ def test_learned_gain_on_batch_workload(dqc, tmp_path):
     print(f"learned gain over FIFO (wall): {fifo / learned:.3f}x", flush=True)
 
 
-def test_backpressure_below_peak(dqc, manifest, tmp_path, monkeypatch):
+BACKPRESSURE_VARIANTS = [
+    ("fifo", "SCHED_FIFO", 0, (0, 0)),
+    ("size_split", "SCHED_SIZE", 0, (2, 3)),
+    ("learned_split", "SCHED_LEARNED", 0, (2, 3)),
+    ("size_split_resident", "SCHED_SIZE", 4, (2, 3)),
+    ("learned_split_resident", "SCHED_LEARNED", 4, (2, 3)),
+]
+
+
+@pytest.mark.parametrize("variant", BACKPRESSURE_VARIANTS, ids=[v[0] for v in BACKPRESSURE_VARIANTS])
+def test_backpressure_below_peak(dqc, manifest, tmp_path, monkeypatch, variant):
     """Each case is run once with the default stack to measure its peak stack use, then
     with a stack of three quarters of that peak: workgroups that find it nearly full defer
     the rest of their task (it runs once the main stack is empty). Every run either gives
     the golden bytes and counts or stops cleanly with DQC_E_CAPACITY, and the deferral
-    path ran in completed runs."""
+    path ran in completed runs. The split variants cut every task of 2 or more branches
+    into pieces of 3, so a deferral can meet a split pass between two chunk starts (the
+    counts would then show any step run twice); host-driven and device-resident."""
+    from cudqc_amd import learn
+    name, sched_name, resident, split = variant
+    sched = getattr(dqc, sched_name)
+    model = learn.load_model() if sched == dqc.SCHED_LEARNED else None
     monkeypatch.setenv("DQC_SERIAL_ROOT", "1")  # level 0 through the task loop (checked)
     cases = []
     for c in manifest["cases"]:
-        cfg = config(dqc, dqc.SCHED_FIFO)
+        cfg = config(dqc, sched, resident=resident, split=split)
         with dqc.Engine(cfg) as eng:
             eng.prepare(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"])
+            if model is not None:
+                eng.set_cost_model(model)
             eng.run()
             st = eng.stats()
         if st.peak_stack_records >= 4 * max(c["n_condensed"], 1) * 8:
@@ -150,16 +168,16 @@ def test_backpressure_below_peak(dqc, manifest, tmp_path, monkeypatch):
     assert len(cases) >= 5
     done, bad, clean = [], [], 0
     for c, pv, pt in cases:
-        cfg = config(dqc, dqc.SCHED_FIFO)
+        cfg = config(dqc, sched, resident=resident, split=split)
         cfg.spill_vertices = pv * 3 // 4
         cfg.spill_tasks = max(pt * 3 // 4, 64)
         pb = []
         try:
-            bad += run_golden(dqc, {"cases": [c]}, tmp_path, cfg, pushbacks=pb)
+            bad += run_golden(dqc, {"cases": [c]}, tmp_path, cfg, model=model, pushbacks=pb)
             done.append((c["name"], pb[0]))
         except dqc.DQCError as e:  # a stack too small for the search fails cleanly
             assert "DQC_E_CAPACITY" in str(e), e
             clean += 1
-    print(f"completed (case, deferred): {done}; clean capacity errors: {clean}", flush=True)
+    print(f"{name}: completed (case, deferred): {done}; clean capacity errors: {clean}", flush=True)
     assert not bad, bad
     assert any(n > 0 for _, n in done)

@@ -101,3 +101,71 @@ def test_search_rejects_bad_arguments(dqc, tmp_path):
             dqc.search_file(str(p), go, gi, mn)
     with pytest.raises(dqc.DQCError, match="DQC_E_IO"):
         dqc.search_file(str(tmp_path / "missing.txt"), 0.8, 0.8, 3)
+
+
+def test_min_degrees_closed_form(dqc):
+    """dqc_min_degrees = h_calculate_minimum_degrees: n + 1 entries of ceil(gamma (k - 1))
+    in IEEE double (0.56 * 25 = 14.000000000000002 gives 15)."""
+    import math
+    for g in (0.5, 0.56, 0.8, 0.9, 0.95, 1.0):
+        assert dqc.min_degrees(g, 60) == [0] + [math.ceil(g * (k - 1)) for k in range(1, 61)]
+    assert dqc.min_degrees(0.56, 30)[26] == 15
+    assert dqc.min_degrees(0.9, 0) == [0]
+    lib = dqc.load()
+    arr = (ctypes.c_int32 * 4)()
+    assert lib.dqc_min_degrees(0.9, -1, arr) == 3 and lib.dqc_min_degrees(1.5, 3, arr) == 3
+
+
+def test_binary_cache_round_trip(dqc, tmp_path):
+    """The binary graph cache gives back exactly what the text loader (and the two-hop
+    builder) produce, with or without stored two-hop lists; level 0 read from it equals
+    level 0 from the text file."""
+    rng = random.Random(5)
+    for t in range(12):
+        path = str(tmp_path / f"g{t}.txt")
+        rand_graph(path, rng, rng.randint(1, 60), rng.choice([0.05, 0.2, 0.5]))
+        ref = dqc.load_graph(path)
+        for two in (True, False):
+            b = str(tmp_path / f"g{t}_{int(two)}.bin")
+            dqc.save_graph(path, b, with_twohop=two)
+            assert dqc.load_graph(b) == ref, (t, two)
+            nt = dqc.load_graph(b, with_twohop=False)
+            assert nt["out"] == ref["out"] and nt["in"] == ref["in"]
+    g = str(tmp_path / "w3.txt")
+    dqc.generate(3, g, 4)
+    b = str(tmp_path / "w3.bin")
+    dqc.save_graph(g, b)
+    assert dqc.level0(b, 0.85, 0.85, 10) == dqc.level0(g, 0.85, 0.85, 10)
+
+
+def test_binary_cache_rejects_damage(dqc, tmp_path):
+    g = str(tmp_path / "w3.txt")
+    dqc.generate(3, g, 1)
+    b = tmp_path / "w3.bin"
+    dqc.save_graph(g, str(b))
+    data = bytearray(b.read_bytes())
+    bad = tmp_path / "bad.bin"
+    for pos in (30, len(data) // 2, len(data) - 3):  # flipped bytes: checksum (or bounds) fail
+        d = bytearray(data)
+        d[pos] ^= 0x41
+        bad.write_bytes(bytes(d))
+        with pytest.raises(dqc.DQCError):
+            dqc.load_graph(str(bad))
+    bad.write_bytes(bytes(data[:len(data) // 3]))  # truncated
+    with pytest.raises(dqc.DQCError):
+        dqc.load_graph(str(bad))
+
+
+def test_search_graph_checks_its_input(dqc):
+    """dqc_search_graph refuses a CSR that breaks the loader's invariants (unsorted or
+    duplicate ids, self loops, ids out of range, in-lists not the transpose), with
+    DQC_E_ARG and without touching the GPU; a graph with no level-0 survivor returns no
+    set, also without the GPU."""
+    out = [[1, 2], [2], [0]]
+    inn = [[2], [0], [0, 1]]
+    for o, i in (([[2, 1], [2], [0]], inn), ([[1, 1], [2], [0]], inn), ([[0, 1], [2], [0]], inn),
+                 ([[1, 3], [2], [0]], inn), (out, [[2], [0], [1]])):
+        with pytest.raises(dqc.DQCError, match="DQC_E_ARG"):
+            dqc.search_graph(o, i, None, None, None, 0.9, 0.9, 2)
+    sets, st = dqc.search_graph(out, inn, None, None, None, 0.9, 0.9, 3)
+    assert sets == [] and st.n_condensed == 0 and st.maximal_cliques == 0
