@@ -46,6 +46,7 @@ class Config(ctypes.Structure):
         ("host_addfail_check", ctypes.c_int32),
         ("split_branches", ctypes.c_int32), ("split_chunk", ctypes.c_int32),
         ("schedule", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
+        ("root_branches", ctypes.c_int32),
     ]
 
 

@@ -14,8 +14,9 @@
 //     child-candidate, in-L(v), critical. Every "is this neighbour in the task" test
 //     of the reference's vertex_order_map (src/device_kernels.cu:433-435) becomes one
 //     LDS bit test;
-//   * HBM arrays indexed by condensed id for the parent's and the child's degrees, so
-//     scatters need no position lookup;
+//   * HBM arrays indexed by condensed id for the parent's and the child's degrees (one
+//     16-byte record of four degrees per id for each), so scatters need no position
+//     lookup;
 //   * a bitonic sort of each child on a packed 64-bit enumeration-order key, in LDS for
 //     children of up to SORT_N vertices and in the workgroup's HBM area above that (LDS
 //     is kept small so that eight workgroups share a CU).
@@ -154,6 +155,7 @@ struct KParams {
     int32_t kloc;               // largest vertex count of one graph (scratch arrays, bitmaps)
     int32_t sum_shift;          // least log2 ids per live-summary bit (HBM-bitmap variant)
     uint32_t max_depth;         // children deeper than this are counted, not pushed (0 = no bound)
+    int32_t root_steps;         // > 0: a level-0 task runs only its loop steps s < root_steps
     unsigned long long maxtot;  // largest task of the batch (~0ull: q->maxtot, device-resident steps)
     const int32_t* gbase;       // batched graphs: first id of each graph (multiples of 32); null = one graph
 };
@@ -192,6 +194,15 @@ __device__ __forceinline__ bool contains(const int32_t* a, unsigned long long lo
     unsigned long long l = lo + cnt_lt(a, lo, hi, x);
     return l < hi && a[l] == x;
 }
+
+// One field of the id-indexed degree records. A vertex's four degrees (out_mem,
+// out_can, in_mem, in_can) share one 16-byte slot, so the accesses that read or write
+// all four (copies into the child, validity tests, child records) touch one cache line
+// per vertex instead of four.
+struct Field {
+    uint32_t* p;
+    __device__ __forceinline__ uint32_t& operator[](long long x) const { return p[4 * x]; }
+};
 
 // All functions below are executed by every thread of the workgroup with uniform
 // control flow; `W` bundles the per-workgroup pointers and uniform scalars.
@@ -233,15 +244,19 @@ struct W {
         cur_gid = gid;
         gb = P->gbase ? P->gbase[gid] : 0;
         const long long k = P->kloc;
-        uint32_t* b = vbase - gb;
-        pom = b; poc = b + k; pim = b + 2 * k; pic = b + 3 * k; pl2 = b + 4 * k;
-        com = b + 5 * k; coc = b + 6 * k; cim = b + 7 * k; cic = b + 8 * k; adjc = b + 9 * k;
+        uint32_t* pr = vbase - 4ll * gb;          // parent degree records
+        uint32_t* cr = vbase + 4 * k - 4ll * gb;  // child degree records
+        pom = Field{pr}; poc = Field{pr + 1}; pim = Field{pr + 2}; pic = Field{pr + 3};
+        com = Field{cr}; coc = Field{cr + 1}; cim = Field{cr + 2}; cic = Field{cr + 3};
+        pl2 = vbase + 8 * k - gb;
+        adjc = vbase + 9 * k - gb;
     }
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
     uint32_t nbranch;             // branches run for the current task / piece
     uint32_t launch_steps;        // stack checks this workgroup passed in this launch
-    uint32_t *pom, *poc, *pim, *pic, *pl2, *com, *coc, *cim, *cic, *adjc;
+    Field pom, poc, pim, pic, com, coc, cim, cic;  // parent and child degrees (4 per id)
+    uint32_t *pl2, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
     unsigned long long* hsort;  // HBM sort keys for children above SORT_N (2k entries)
     unsigned long long bytes;  // per-thread traffic tally (CSR + records)
@@ -1199,7 +1214,8 @@ struct W {
         if (tid == 0 && !piece) atomicAdd(&P->g->tasks, 1ull);
         __syncthreads();
         int tot = T, nc = T - nm;
-        const int expansions = piece ? min(d.iters, nc) : nc;
+        int expansions = piece ? min(d.iters, nc) : nc;
+        if (mode == MODE_HOST_L0 && P->root_steps > 0) expansions = min(expansions, ncov + P->root_steps);
         // the batch scheduler marks the tasks to split (host side, within a record budget)
         const int steps = expansions - ncov;
         const int chunk = max(P->split_c, (steps + MAX_PIECES - 1) / MAX_PIECES);
@@ -1636,8 +1652,8 @@ __global__ void refill_kernel(const TaskDesc* sp_desc, const dqc::TaskVertex* sp
 // bit0 = the removal at this step skips the branch (success 2), bit1 = lookahead holds,
 // bit2 = fewer than min_size vertices remain (the loop stops). Uses the same counts as
 // branch_root: after s removals the live set is [0, k - s).
-__global__ void root_flags_kernel(KParams P, const dqc::TaskVertex* root, int k, int ncov, uint8_t* flags) {
-    const int S = k - ncov;
+__global__ void root_flags_kernel(KParams P, const dqc::TaskVertex* root, int k, int S, uint8_t* flags) {
+    // S: the steps to classify, at most k - ncov (fewer in a bounded unit of work)
     const int mo1 = P.mdo[P.min_size], mi1 = P.mdi[P.min_size];  // gmd(nm + 1) with nm = 0
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
         const int tot = k - s;
@@ -1911,7 +1927,8 @@ static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint
             (m_two && !hip_ok(h, hipMemcpy(h->d_ltn, ltn.data(), sizeof(int32_t) * m_two, hipMemcpyHostToDevice), "upload")))
             return DQC_E_HIP;
     }
-    // per-workgroup scratch: 10 id-indexed value arrays + 8 lists, k entries each, the
+    // per-workgroup scratch: 10k words of id-indexed values (parent and child degree
+    // records of 4 words per id, lvl2adj, critical counts) + 8 lists of k entries, the
     // HBM sort area (2k 64-bit keys: a child has at most k vertices, padded to a power of
     // 2) and, for large graphs, the six id bitmaps (all words even, so 8-byte aligned)
     const unsigned long long kk = (unsigned long long)h->kloc, kwords = (kk + 31) / 32;
@@ -2319,6 +2336,7 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.kloc = h->kloc;
     P.gbase = h->d_gbase;
     P.max_depth = (uint32_t)std::max(0, h->cfg.max_depth);
+    P.root_steps = std::max(0, h->cfg.root_branches);
     P.maxtot = std::max<unsigned long long>(h->batch_maxtot, 1);
     return P;
 }
@@ -2399,10 +2417,12 @@ static dqc_status split_root(dqc_handle* h) {
     if (!hip_ok(h, hipStreamSynchronize(h->stream), "pop")) return DQC_E_HIP;
     TaskDesc d;
     if (!hip_ok(h, hipMemcpy(&d, h->d_bdesc, sizeof d, hipMemcpyDeviceToHost), "root read")) return DQC_E_HIP;
-    const int k = d.tot, ncov = d.ncov, S = k - ncov;
+    const int k = d.tot, ncov = d.ncov;
+    // a bounded unit of work: only the first root_branches loop steps (dqc_config)
+    const int S = h->cfg.root_branches > 0 ? std::min(k - ncov, h->cfg.root_branches) : k - ncov;
     KParams P = make_params(h, 0);
     if (S > 0) {
-        root_flags_kernel<<<(S + 255) / 256, 256, 0, h->stream>>>(P, h->d_bvtx + d.off, k, ncov, h->d_flags);
+        root_flags_kernel<<<(S + 255) / 256, 256, 0, h->stream>>>(P, h->d_bvtx + d.off, k, S, h->d_flags);
         if (!hip_ok(h, hipGetLastError(), "flags launch") || !hip_ok(h, hipStreamSynchronize(h->stream), "flags kernel"))
             return DQC_E_HIP;
     }

@@ -53,6 +53,10 @@ typedef struct {
                                   device, resident_steps per replay of one captured HIP
                                   graph, with no host round trip between them;
                                   0 = host-driven steps                               */
+    int32_t root_branches;     /* > 0: only the level-0 loop steps s < root_branches of
+                                  a single graph run (with max_depth, a fixed bounded
+                                  unit of work: the first level-0 branches and their
+                                  subtrees down to that depth); 0 = all              */
 } dqc_config;
 
 typedef struct {
