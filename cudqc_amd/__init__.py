@@ -23,6 +23,7 @@ EXPORTS = [
     "dqc_task_features", "dqc_mlp_forward", "dqc_mlp_gradient", "dqc_mlp_train", "dqc_set_cost_model",
     "dqc_upload_graph_batch", "dqc_push_graph_root", "dqc_fetch_batch_cliques", "dqc_prepare_batch",
     "dqc_search_batch", "dqc_min_degrees", "dqc_search_graph", "dqc_prepare_graph", "dqc_save_graph",
+    "dqc_expand_noalloc",
 ]
 
 MLP_IN, MLP_HIDDEN = 7, 64

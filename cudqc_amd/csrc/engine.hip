@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2702,3 +2703,6 @@ dqc_status dqc_get_stats(const dqc_handle* h, dqc_stats* st) {
 }
 
 }  // extern "C"
+
+// the allocation-free entry point launches the kernels above
+#include "noalloc.hip"

@@ -301,6 +301,67 @@ int64_t dqc_remove_non_max(const char* in_path, const char* out_path);
  * one of the 256 graphs of workload 3. Returns the vertex count or <0. */
 int32_t dqc_generate(int32_t config, int32_t graph_index, const char* path);
 
+/* ---- Allocation-free batch expansion (no counterpart in the reference) ------------
+ * For callers that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. One expansion launch of a batch of
+ * tasks, as dqc_step runs it, with every input and output in the caller's device memory
+ * and the kernel scratch (launch counters, per-workgroup degree records and lists) in a
+ * static device array of the library (about 92 MB per GPU, held until the process
+ * ends). Nothing is allocated and nothing waits for the host: the launches are queued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the outcome lands in `counters`.
+ * Limits: condensed graphs of at most DQC_NOALLOC_MAX_VERTICES vertices (larger ones:
+ * DQC_E_UNSUPPORTED), DQC_NOALLOC_WORKGROUPS workgroups. Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU race, so order them (one stream, or
+ * synchronise between them). */
+#define DQC_NOALLOC_MAX_VERTICES 4096
+#define DQC_NOALLOC_WORKGROUPS 256
+#define DQC_NOALLOC_COUNTERS 9
+
+/* Task descriptor in device memory (56 bytes): records [offset, offset + size) of the
+ * record array, members first, then `covered` covered vertices (level-0 tasks), then
+ * candidates; mode 0 / 1 / 2 as dqc_push_task; steps = size - members - covered for a
+ * whole task (flags 0); flags & 1 marks a piece (a deferred rest of a task, as this call
+ * writes them: it runs `steps` loop steps from its state, flags & 2 = its first step
+ * removes). id, parent, depth, graph and the degree extremes are carried along. */
+typedef struct {
+    uint64_t offset;
+    int32_t size, members, covered, mode, steps, flags;
+    uint32_t id, parent;
+    uint16_t depth, graph, dmin_out, dmax_out, dmin_in, dmax_in;
+} dqc_task_desc;
+
+/* A condensed graph and its tables in device memory (as dqc_upload_graph and
+ * dqc_set_tables take them on the host); table_len >= 2 (max(n, min_size) + 2) + 2. */
+typedef struct {
+    int32_t n;
+    const uint64_t* out_off;
+    const int32_t* out_nbr;
+    const uint64_t* in_off;
+    const int32_t* in_nbr;
+    const uint64_t* two_off;
+    const int32_t* two_nbr;
+    const int32_t *md_out, *md_in, *ub_out, *ub_in;
+    int32_t table_len, min_size;
+    int32_t host_addfail_check; /* as dqc_config.host_addfail_check                   */
+} dqc_device_graph;
+
+/* Expands tasks[0 .. ntasks) (records: 6 uint32 per vertex, nrecords in all; the
+ * largest task has max_task_size vertices) one level. Children go to out_tasks /
+ * out_records from index 0 up; when those buffers come close to full, a task's
+ * remaining steps are written as a piece from the top down (descriptors
+ * [out_task_cap - n, out_task_cap), records likewise) instead of running now: feed both
+ * back as tasks. Cliques: clique_ids, clique_meta ((first id << 24) | length each).
+ * counters (device, DQC_NOALLOC_COUNTERS uint64) receives {children, child records,
+ * pieces, piece records, tasks expanded, branches, cliques, clique ids, error bits
+ * (1 tasks did not fit, 2 cliques did not fit, 4 too many members)}. Returns DQC_OK once
+ * the launches are queued. A variant of dqc_step (one batch, one launch). */
+dqc_status dqc_expand_noalloc(const dqc_device_graph* graph, const dqc_task_desc* tasks,
+                              const uint32_t* records, int64_t ntasks, int64_t nrecords,
+                              int32_t max_task_size, dqc_task_desc* out_tasks, uint32_t* out_records,
+                              int64_t out_task_cap, int64_t out_record_cap, int32_t* clique_ids,
+                              uint64_t* clique_meta, int64_t clique_cap, int64_t clique_id_cap,
+                              uint64_t* counters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
