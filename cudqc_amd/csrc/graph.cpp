@@ -229,8 +229,23 @@ int load_graph_file(const std::string& path, Graph& g, std::string& err, int thr
 // from O u B; w stays in O while it has an arc into I u B; both sides use the
 // previous round's sets). The list is B u O* u I* plus every other x != v that has
 // an arc from O*uB, an arc into O*uB, an arc from I*uB and an arc into I*uB.
-void build_twohop(Graph& g, int threads) {
+namespace {
+void twohop_impl(Graph& g, int threads, const std::vector<uint8_t>* src, const std::vector<uint8_t>* dst,
+                 std::vector<uint32_t>* full_len);
+}
+
+void build_twohop(Graph& g, int threads) { twohop_impl(g, threads, nullptr, nullptr, nullptr); }
+
+void build_twohop_restricted(Graph& g, int threads, const std::vector<uint8_t>& src, const std::vector<uint8_t>& dst,
+                             std::vector<uint32_t>& full_len) {
+    twohop_impl(g, threads, &src, &dst, &full_len);
+}
+
+namespace {
+void twohop_impl(Graph& g, int threads, const std::vector<uint8_t>* src, const std::vector<uint8_t>* dst,
+                 std::vector<uint32_t>* full_len) {
     const int32_t n = g.n;
+    if (full_len) full_len->assign((size_t)n, 0);
     const uint64_t* oo = g.out.off.data();
     const int32_t* on = g.out.nbr.data();
     const uint64_t* io = g.in.off.data();
@@ -258,10 +273,11 @@ void build_twohop(Graph& g, int threads) {
                 if (nchunks >= 10 && d % (nchunks / 10) == 0)
                     std::fprintf(stderr, "dqc: two-hop lists %d/%d chunks\n", d, nchunks);
             }
-            std::vector<int32_t>& dst = chunk_lists[ch];
+            std::vector<int32_t>& out = chunk_lists[ch];
             int32_t v0 = ch * chunk, v1 = std::min(n, v0 + chunk);
             for (int32_t v = v0; v < v1; ++v) {
                 if (oo[v + 1] == oo[v] || io[v + 1] == io[v]) continue;
+                if (src && !(*src)[v]) continue;
                 B.clear(); O.clear(); I.clear();
                 for (uint64_t k = oo[v]; k < oo[v + 1]; ++k) nb[on[k]] |= 1;
                 for (uint64_t k = io[v]; k < io[v + 1]; ++k) nb[in[k]] |= 2;
@@ -326,9 +342,13 @@ void build_twohop(Graph& g, int threads) {
                 for (int32_t w : B) inl[w] = 0;
                 for (int32_t w : O) inl[w] = 0;
                 for (int32_t w : I) inl[w] = 0;
+                if (full_len) {  // the whole length, then only the flagged entries are kept
+                    (*full_len)[v] = (uint32_t)lst.size();
+                    lst.erase(std::remove_if(lst.begin(), lst.end(), [&](int32_t x) { return !(*dst)[x]; }), lst.end());
+                }
                 std::sort(lst.begin(), lst.end());
                 cnt[v] = (uint32_t)lst.size();
-                dst.insert(dst.end(), lst.begin(), lst.end());
+                out.insert(out.end(), lst.begin(), lst.end());
             }
         }
     }
@@ -342,6 +362,7 @@ void build_twohop(Graph& g, int threads) {
         std::copy(chunk_lists[ch].begin(), chunk_lists[ch].end(), g.two.nbr.begin() + g.two.off[v0]);
     }
 }
+}  // namespace
 
 std::vector<int32_t> min_degree_table(double gamma, int len) {
     std::vector<int32_t> md((size_t)len, 0);
@@ -377,8 +398,19 @@ inline bool l0_before(const L0V& a, const L0V& b) {
 
 // reference src/host_functions.cu:461-686 (h_initialize_tasks) and :690-934
 // (h_condense_graph).
+void level0_from_loaded(Graph& g, const int32_t* md_out, const int32_t* md_in, int min_size, int threads,
+                        Level0& out) {
+    const uint32_t tho = (uint32_t)md_out[min_size], thi = (uint32_t)md_in[min_size];
+    std::vector<uint8_t> flag((size_t)g.n, 0);
+    for (int32_t v = 0; v < g.n; ++v)
+        flag[v] = (g.out.off[v + 1] - g.out.off[v] >= tho) && (g.in.off[v + 1] - g.in.off[v] >= thi);
+    std::vector<uint32_t> full;
+    build_twohop_restricted(g, threads, flag, flag, full);
+    level0_condense(g, md_out, md_in, min_size, out, &full);
+}
+
 void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int min_size,
-                     Level0& res) {
+                     Level0& res, const std::vector<uint32_t>* l2_full) {
     const int32_t n = g.n;
     const uint32_t tho = (uint32_t)md_out[min_size], thi = (uint32_t)md_in[min_size];
     const bool progress = std::getenv("DQC_PROGRESS") != nullptr;  // phase lines for large graphs
@@ -391,7 +423,7 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
         v[i].id = (uint32_t)i;
         v[i].oc = (uint32_t)(g.out.off[i + 1] - g.out.off[i]);
         v[i].ic = (uint32_t)(g.in.off[i + 1] - g.in.off[i]);
-        v[i].l2 = (uint32_t)(g.two.off[i + 1] - g.two.off[i]);
+        v[i].l2 = l2_full ? (*l2_full)[i] : (uint32_t)(g.two.off[i + 1] - g.two.off[i]);
         bool ok = v[i].oc >= tho && v[i].ic >= thi && v[i].l2 >= (uint32_t)(min_size - 1);
         v[i].label = ok ? 0 : -1;
         (ok ? alive : dropped).push_back(i);

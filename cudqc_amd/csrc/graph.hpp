@@ -37,6 +37,16 @@ int load_text(const std::string& path, Graph& g, std::string& err);
 // Directed two-hop lists (reference src/common.cpp:111-509 semantics).
 void build_twohop(Graph& g, int threads);
 
+// Two-hop lists for the search path. Level 0 reads a vertex's two-hop list only if the
+// vertex passes the degree part of its initial filter (src/host_functions.cu:504-505),
+// and after condensing keeps only entries that survive it; so lists are built only for
+// vertices with src[v] set, and only their entries x with dst[x] set are stored (both
+// flags: out-degree >= md_out[min] and in-degree >= md_in[min], a superset of the
+// survivors). full_len[v] gets the length of v's whole list (the filter and the
+// enumeration order use it); vertices not built get 0.
+void build_twohop_restricted(Graph& g, int threads, const std::vector<uint8_t>& src, const std::vector<uint8_t>& dst,
+                             std::vector<uint32_t>& full_len);
+
 // Binary graph file, a cache of what the text loader (and optionally the two-hop
 // builder) produces, so that a large graph is parsed and its two-hop lists built once.
 // Layout (little-endian): the 8 bytes "DQCGRPH1", int32 n, int32 flags (bit 0: two-hop
@@ -69,8 +79,14 @@ struct Level0 {
     int32_t ncov = 0;              // covered vertices right after the (zero) members
     std::vector<int32_t> orig_of;  // input id of each condensed id
 };
+// l2_full (optional): whole two-hop list lengths when g.two holds restricted lists
+// (build_twohop_restricted); null = the lengths of g.two's lists.
 void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int min_size,
-                     Level0& out);
+                     Level0& out, const std::vector<uint32_t>* l2_full = nullptr);
+// The search path's preprocessing of a loaded graph without two-hop lists: degree
+// flags from the tables, restricted two-hop lists, then level 0 + condense.
+void level0_from_loaded(Graph& g, const int32_t* md_out, const int32_t* md_in, int min_size, int threads,
+                        Level0& out);
 
 // The maximal-set filter of reference src/Quick_rmnonmax.cpp (RemoveNonMax):
 // reads "<len> ids..." records, writes the maximal distinct sets, each ascending,

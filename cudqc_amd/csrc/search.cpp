@@ -103,9 +103,17 @@ bool adopt_csr(int32_t n, const uint64_t* out_off, const int32_t* out_nbr, const
     if (two_off) {
         if (!copy(two_off, two_nbr, g.two, "two-hop")) return false;
     } else {
-        dqc::build_twohop(g, omp_get_max_threads());
+        g.two.off.assign((size_t)n + 1, 0);  // built by level0_impl, for the candidates only
+        g.two.nbr.clear();
     }
     return true;
+}
+
+// Text or binary graph for the search path; has_two: the file held two-hop lists.
+int load_for_search(const char* path, dqc::Graph& g, std::string& err, bool& has_two) {
+    has_two = false;
+    if (dqc::is_binary_graph(path)) return dqc::load_binary(path, g, err, &has_two) ? 0 : 2;
+    return dqc::load_text(path, g, err);
 }
 
 // Tables and level 0 + condense on g, which becomes the condensed graph (as h_search's
@@ -115,14 +123,17 @@ struct Prepared {
     std::vector<int32_t> mdo, mdi, ubo, ubi;
     dqc::Level0 l0;
 };
-void level0_impl(dqc::Graph& g, const int32_t* md_out, const int32_t* md_in, int32_t md_len, double go, double gi,
-                 int32_t min_size, Prepared& p) {
+// has_two: g holds complete two-hop lists; otherwise they are built here for the
+// vertices level 0 can keep (dqc::level0_from_loaded), with identical results.
+void level0_impl(dqc::Graph& g, bool has_two, const int32_t* md_out, const int32_t* md_in, int32_t md_len, double go,
+                 double gi, int32_t min_size, Prepared& p, int threads) {
     p.len = table_len(g.n, min_size);
     p.mdo = extend_table(md_out, md_len, go, p.len);
     p.mdi = extend_table(md_in, md_len, gi, p.len);
     p.ubo = dqc::bound_div_table(go, p.len);
     p.ubi = dqc::bound_div_table(gi, p.len);
-    dqc::level0_condense(g, p.mdo.data(), p.mdi.data(), min_size, p.l0);
+    if (has_two) dqc::level0_condense(g, p.mdo.data(), p.mdi.data(), min_size, p.l0);
+    else dqc::level0_from_loaded(g, p.mdo.data(), p.mdi.data(), min_size, threads, p.l0);
 }
 
 // Upload of the condensed graph, the tables and the level-0 task into an open handle.
@@ -139,12 +150,12 @@ dqc_status upload_impl(dqc_handle* h, const dqc::Graph& g, const Prepared& p, in
 // The whole search over a loaded graph (with two-hop lists): level 0 on the host, then
 // (when anything survives) one handle and the expansion to completion, then the raw
 // file, the maximal filter, the maximal file and the sink.
-dqc_status search_impl(dqc::Graph& g, double go, double gi, int32_t min_size, const int32_t* md_out,
+dqc_status search_impl(dqc::Graph& g, bool has_two, double go, double gi, int32_t min_size, const int32_t* md_out,
                        const int32_t* md_in, int32_t md_len, const char* raw_path, const char* max_path,
                        dqc_set_sink sink, void* sink_ctx, const dqc_config* cfg, int64_t* max_cliques,
                        dqc_stats* st, int32_t* orig_of_out, std::chrono::steady_clock::time_point t_start) {
     Prepared p;
-    level0_impl(g, md_out, md_in, md_len, go, gi, min_size, p);
+    level0_impl(g, has_two, md_out, md_in, md_len, go, gi, min_size, p, omp_get_max_threads());
     dqc_stats stats{};
     std::vector<std::vector<int32_t>> sets;
     if (!p.l0.task.empty()) {
@@ -262,12 +273,12 @@ dqc_status dqc_level0(const char* graph_path, double out_gamma, double in_gamma,
     if (!graph_path || !n_condensed || !ncov || min_size <= 1) return DQC_E_ARG;
     dqc::Graph g;
     std::string err;
-    int rc = dqc::load_graph_file(graph_path, g, err, omp_get_max_threads());
+    bool has_two = false;
+    int rc = load_for_search(graph_path, g, err, has_two);
     if (rc) return rc == 1 ? DQC_E_IO : DQC_E_ARG;
-    const int len = table_len(g.n, min_size);
-    auto mdo = dqc::min_degree_table(out_gamma, len), mdi = dqc::min_degree_table(in_gamma, len);
-    dqc::Level0 l0;
-    dqc::level0_condense(g, mdo.data(), mdi.data(), min_size, l0);
+    Prepared p;
+    level0_impl(g, has_two, nullptr, nullptr, 0, out_gamma, in_gamma, min_size, p, omp_get_max_threads());
+    const dqc::Level0& l0 = p.l0;
     *n_condensed = (int32_t)l0.task.size();
     *ncov = l0.ncov;
     if (records) {
@@ -286,10 +297,11 @@ dqc_status dqc_prepare_file(dqc_handle* h, const char* graph_path, double out_ga
     if (!h || !graph_path || !gamma_ok(out_gamma) || !gamma_ok(in_gamma) || min_size <= 1) return DQC_E_ARG;
     dqc::Graph g;
     std::string err;
-    int rc = dqc::load_graph_file(graph_path, g, err, omp_get_max_threads());
+    bool has_two = false;
+    int rc = load_for_search(graph_path, g, err, has_two);
     if (rc) return rc == 1 ? DQC_E_IO : DQC_E_ARG;
     Prepared p;
-    level0_impl(g, nullptr, nullptr, 0, out_gamma, in_gamma, min_size, p);
+    level0_impl(g, has_two, nullptr, nullptr, 0, out_gamma, in_gamma, min_size, p, omp_get_max_threads());
     if (n_condensed) *n_condensed = (int32_t)p.l0.task.size();
     return upload_impl(h, g, p, min_size);
 }
@@ -309,7 +321,7 @@ dqc_status dqc_prepare_graph(dqc_handle* h, int32_t n, const uint64_t* out_off, 
         return DQC_E_ARG;
     }
     Prepared p;
-    level0_impl(g, md_out, md_in, md_len, out_gamma, in_gamma, min_size, p);
+    level0_impl(g, two_off != nullptr, md_out, md_in, md_len, out_gamma, in_gamma, min_size, p, omp_get_max_threads());
     if (n_condensed) *n_condensed = (int32_t)p.l0.task.size();
     if (orig_of && !p.l0.orig_of.empty())
         std::memcpy(orig_of, p.l0.orig_of.data(), sizeof(int32_t) * p.l0.orig_of.size());
@@ -329,11 +341,13 @@ dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int3
     for (int32_t i = 0; i < ngraphs; ++i) {
         std::string err;
         if (!graph_paths[i]) { rcs[i] = 2; continue; }
-        rcs[i] = dqc::load_graph_file(graph_paths[i], g[i], err, 1);
+        bool has_two = false;
+        rcs[i] = load_for_search(graph_paths[i], g[i], err, has_two);
         if (rcs[i]) continue;
-        lens[i] = table_len(g[i].n, min_size);
-        auto mdo = dqc::min_degree_table(out_gamma, lens[i]), mdi = dqc::min_degree_table(in_gamma, lens[i]);
-        dqc::level0_condense(g[i], mdo.data(), mdi.data(), min_size, l0[i]);
+        Prepared p;
+        level0_impl(g[i], has_two, nullptr, nullptr, 0, out_gamma, in_gamma, min_size, p, 1);
+        lens[i] = p.len;
+        l0[i] = std::move(p.l0);
     }
     int len = 2;
     for (int32_t i = 0; i < ngraphs; ++i) {
@@ -428,13 +442,14 @@ dqc_status dqc_search_file(const char* graph_path, double out_gamma, double in_g
     auto tp = std::chrono::steady_clock::now();
     dqc::Graph g;
     std::string err;
-    int rc = dqc::load_graph_file(graph_path, g, err, omp_get_max_threads());
+    bool has_two = false;
+    int rc = load_for_search(graph_path, g, err, has_two);
     if (rc) {
         std::fprintf(stderr, "dqc: %s\n", err.c_str());
         return rc == 1 ? DQC_E_IO : DQC_E_ARG;
     }
-    return search_impl(g, out_gamma, in_gamma, min_size, nullptr, nullptr, 0, raw_path, max_path, nullptr, nullptr,
-                       cfg, max_cliques, st, nullptr, tp);
+    return search_impl(g, has_two, out_gamma, in_gamma, min_size, nullptr, nullptr, 0, raw_path, max_path, nullptr,
+                       nullptr, cfg, max_cliques, st, nullptr, tp);
 }
 
 dqc_status dqc_search_graph(int32_t n, const uint64_t* out_off, const int32_t* out_nbr, const uint64_t* in_off,
@@ -453,8 +468,8 @@ dqc_status dqc_search_graph(int32_t n, const uint64_t* out_off, const int32_t* o
         std::fprintf(stderr, "dqc: %s\n", err.c_str());
         return DQC_E_ARG;
     }
-    return search_impl(g, out_gamma, in_gamma, min_size, md_out, md_in, md_len, raw_path, max_path, sink, sink_ctx,
-                       cfg, max_cliques, st, orig_of, tp);
+    return search_impl(g, two_off != nullptr, out_gamma, in_gamma, min_size, md_out, md_in, md_len, raw_path, max_path,
+                       sink, sink_ctx, cfg, max_cliques, st, orig_of, tp);
 }
 
 }  // extern "C"

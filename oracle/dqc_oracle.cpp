@@ -578,7 +578,9 @@ struct Expander {
 
     // One task's branch loop: reference src/host_functions.cu:987-1134 (host levels)
     // and src/device_kernels.cu:25-173 (device levels).
-    void expand(const Rec* task, int tot, Mode mode) {
+    // max_steps > 0: only the loop steps j < ncov + max_steps run (a bounded prefix of
+    // the same loop, the library's root_branches unit of work).
+    void expand(const Rec* task, int tot, Mode mode, int max_steps = 0) {
         ensure(tot);
         sink->st.tasks++;
         sink->st.task_hist[31 - __builtin_clz((unsigned)tot | 1)]++;
@@ -591,7 +593,7 @@ struct Expander {
         if (mode != DEVICE)
             while (nm + ncov < tot && rv[nm + ncov].label == 2) ++ncov;
         int nc = tot - nm;
-        const int expansions = nc;
+        const int expansions = max_steps > 0 ? std::min(nc, ncov + max_steps) : nc;
         std::vector<Rec> child(tot);
         for (int j = ncov; j < expansions; ++j) {
             if (j != ncov) {
@@ -1044,7 +1046,7 @@ int orc_root(void* hv, uint32_t* out, int* ncov) {
 // Expands one task one level. Children: *kid_meta gets 2 int64 per child {record offset,
 // size}, members counted by label; *kids the records; cliques as "len, ids..." int32.
 // All outputs malloc'd (orc_free). Returns the number of children.
-int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int mode, uint32_t** kids,
+int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int mode, int max_steps, uint32_t** kids,
                int64_t** kid_meta, int32_t** kid_nm, int32_t** clq, int64_t* clq_len, int64_t* branches) {
     using namespace orc;
     auto* h = (orc_handle*)hv;
@@ -1057,7 +1059,7 @@ int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int m
     Sink sink;
     Expander ex(h->g, h->P);
     ex.sink = &sink;
-    ex.expand(t.data(), nvert, (Mode)mode);
+    ex.expand(t.data(), nvert, (Mode)mode, max_steps);
     size_t tot = 0;
     for (auto& k : sink.kids) tot += k.size();
     *kids = (uint32_t*)std::malloc(sizeof(uint32_t) * 6 * (tot + 1));

@@ -58,7 +58,7 @@ def load():
     lib.orc_root.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]
     PU32 = ctypes.POINTER(ctypes.c_uint32)
     lib.orc_expand.restype = ctypes.c_int
-    lib.orc_expand.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    lib.orc_expand.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.POINTER(PU32), ctypes.POINTER(P64), ctypes.POINTER(P32), ctypes.POINTER(P32),
                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     _lib = lib
@@ -119,8 +119,9 @@ class Tasks:
         self.lib.orc_root(self.h, arr, ctypes.byref(ncov))
         return [tuple(arr[6 * i:6 * i + 6]) for i in range(k)], ncov.value
 
-    def expand(self, records, nm, ncov=0, mode=2):
-        """Returns (children as (records, nm), cliques as sorted id tuples, branches)."""
+    def expand(self, records, nm, ncov=0, mode=2, max_steps=0):
+        """Returns (children as (records, nm), cliques as sorted id tuples, branches);
+        max_steps > 0 runs only the first max_steps loop steps after the covered prefix."""
         flat = (ctypes.c_uint32 * max(6 * len(records), 1))(*[x for r in records for x in r])
         kids = ctypes.POINTER(ctypes.c_uint32)()
         meta = ctypes.POINTER(ctypes.c_int64)()
@@ -128,7 +129,7 @@ class Tasks:
         clq = ctypes.POINTER(ctypes.c_int)()
         clen = ctypes.c_int64()
         br = ctypes.c_int64()
-        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, ctypes.byref(kids), ctypes.byref(meta),
+        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), ctypes.byref(kids), ctypes.byref(meta),
                                 ctypes.byref(knm), ctypes.byref(clq), ctypes.byref(clen), ctypes.byref(br))
         out = []
         for c in range(n):

@@ -60,7 +60,8 @@ int host(int argc, char** argv) {
         return 20;
     }
     dqc::Graph r;
-    if (!dqc::load_binary(argv[7], r, err)) {
+    bool has_two = false;
+    if (!dqc::load_binary(argv[7], r, err, &has_two)) {
         std::fprintf(stderr, "load binary: %s\n", err.c_str());
         return 21;
     }
@@ -77,6 +78,15 @@ int host(int argc, char** argv) {
     h = fnv(h, l0.orig_of.data(), l0.orig_of.size() * sizeof(int32_t));
     std::printf("level0 k=%zu ncov=%d task=%016llx condensed two=%016llx\n", l0.task.size(), l0.ncov,
                 (unsigned long long)h, (unsigned long long)digest(g.two));
+    // the search path: two-hop lists built for the degree candidates only
+    dqc::Level0 l1;
+    dqc::level0_from_loaded(r, mdo.data(), mdi.data(), mn, threads, l1);
+    uint64_t h1 = fnv(1469598103934665603ull, l1.task.data(), l1.task.size() * sizeof(dqc::TaskVertex));
+    h1 = fnv(h1, l1.orig_of.data(), l1.orig_of.size() * sizeof(int32_t));
+    if (h1 != h || digest(r.two) != digest(g.two) || l1.ncov != l0.ncov) {
+        std::fprintf(stderr, "restricted two-hop path differs\n");
+        return 23;
+    }
     return 0;
 }
 

@@ -131,6 +131,10 @@ def test_learned_gain_on_batch_workload(dqc, tmp_path):
     print(f"learned gain over FIFO (wall): {fifo / learned:.3f}x", flush=True)
 
 
+# golden cases with the deepest task stacks (most tasks; peak stack well above one task)
+DEEP_CASES = ["workload3_g0", "workload3_g2", "workload3_g3", "workload3_g8", "rand017", "rand038", "rand069",
+              "rand075", "rand088", "rand100"]
+
 BACKPRESSURE_VARIANTS = [
     ("fifo", "SCHED_FIFO", 0, (0, 0)),
     ("size_split", "SCHED_SIZE", 0, (2, 3)),
@@ -156,6 +160,8 @@ def test_backpressure_below_peak(dqc, manifest, tmp_path, monkeypatch, variant):
     monkeypatch.setenv("DQC_SERIAL_ROOT", "1")  # level 0 through the task loop (checked)
     cases = []
     for c in manifest["cases"]:
+        if c["name"] not in DEEP_CASES:
+            continue
         cfg = config(dqc, sched, resident=resident, split=split)
         with dqc.Engine(cfg) as eng:
             eng.prepare(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"])
