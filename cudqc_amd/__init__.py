@@ -23,7 +23,7 @@ EXPORTS = [
     "dqc_task_features", "dqc_mlp_forward", "dqc_mlp_gradient", "dqc_mlp_train", "dqc_set_cost_model",
     "dqc_upload_graph_batch", "dqc_push_graph_root", "dqc_fetch_batch_cliques", "dqc_prepare_batch",
     "dqc_search_batch", "dqc_min_degrees", "dqc_search_graph", "dqc_prepare_graph", "dqc_save_graph",
-    "dqc_expand_noalloc",
+    "dqc_expand_noalloc", "dqc_batch_size", "dqc_finish_batch",
 ]
 
 MLP_IN, MLP_HIDDEN = 7, 64
@@ -47,7 +47,8 @@ class Config(ctypes.Structure):
         ("host_addfail_check", ctypes.c_int32),
         ("split_branches", ctypes.c_int32), ("split_chunk", ctypes.c_int32),
         ("schedule", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
-        ("root_branches", ctypes.c_int32),
+        ("root_branches", ctypes.c_int32), ("split_pieces", ctypes.c_int32), ("root_first", ctypes.c_int32),
+        ("launch_ms", ctypes.c_int32),
     ]
 
 
@@ -62,7 +63,7 @@ class Stats(ctypes.Structure):
         ("n_condensed", ctypes.c_int32), ("workgroups", ctypes.c_int32),
         ("truncated_children", ctypes.c_int64), ("stack_pushbacks", ctypes.c_int64),
         ("peak_stack_tasks", ctypes.c_int64), ("peak_stack_records", ctypes.c_int64),
-        ("preprocess_ms", ctypes.c_double), ("rmnonmax_ms", ctypes.c_double),
+        ("preprocess_ms", ctypes.c_double), ("rmnonmax_ms", ctypes.c_double), ("time_slices", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -171,6 +172,10 @@ def load():
     lib.dqc_prepare_graph.argtypes = [ctypes.c_void_p, ctypes.c_int32] + csr + [P32, P32, ctypes.c_int32,
                                                                                 ctypes.c_double, ctypes.c_double,
                                                                                 ctypes.c_int32, P32, P32]
+    lib.dqc_finish_batch.restype = ctypes.c_int32
+    lib.dqc_finish_batch.argtypes = [ctypes.c_void_p, PPC, PI64, ctypes.POINTER(Stats)]
+    lib.dqc_batch_size.restype = ctypes.c_int32
+    lib.dqc_batch_size.argtypes = [ctypes.c_void_p, P32]
     lib.dqc_save_graph.restype = ctypes.c_int32
     lib.dqc_save_graph.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     _lib = lib
@@ -399,6 +404,16 @@ class Engine:
             p += sizes[i]
         return out
 
+    def finish_batch(self, max_paths):
+        """dqc_finish_batch: each graph's maximal file (None entries skip); returns
+        (per-graph maximal counts, stats)."""
+        n = len(max_paths)
+        arr = (ctypes.c_char_p * n)(*[_b(p) for p in max_paths])
+        counts = (ctypes.c_int64 * n)()
+        st = Stats()
+        self._check(self.lib.dqc_finish_batch(self.h, arr, counts, ctypes.byref(st)), "dqc_finish_batch")
+        return list(counts), st
+
     def set_task_log(self, capacity):
         self._check(self.lib.dqc_set_task_log(self.h, int(capacity)), "dqc_set_task_log")
 
@@ -418,6 +433,22 @@ class Engine:
         rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
         self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
         return [(meta[9 * i + 4] & 0xFFFFFFFF, meta[9 * i + 5] & 0xFFFFFFFF) for i in range(nt.value)]
+
+    def task_digests(self):
+        """(SHA-1 of the records' bytes, nm, mode) of every task on the stack: a compact
+        form of tasks() for large tasks (records are 6 uint32 each)."""
+        import hashlib
+        nt, nr = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), None, None), "fetch")
+        meta = (ctypes.c_int32 * max(9 * nt.value, 1))()
+        rec = (ctypes.c_uint32 * max(6 * nr.value, 1))()
+        self._check(self.lib.dqc_fetch_tasks(self.h, ctypes.byref(nt), ctypes.byref(nr), meta, rec), "fetch")
+        mv = memoryview(rec).cast("B")
+        out = []
+        for i in range(nt.value):
+            off, tot, nm, mode = meta[9 * i:9 * i + 4]
+            out.append((hashlib.sha1(mv[24 * off:24 * (off + tot)]).hexdigest(), nm, mode))
+        return out
 
     def task_meta(self):
         """(records, nm, mode, ncov) of every task on the stack."""

@@ -70,7 +70,17 @@ enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u, ER
 constexpr int32_t TF_PIECE = 1;         // a piece of a split task: not counted, not split again
 constexpr int32_t TF_REMOVE_FIRST = 2;  // the piece's first step starts with a removal
 constexpr int32_t TF_SPLIT = 4;         // set by the batch scheduler: cut this task into pieces
-constexpr int MAX_PIECES = 64;          // pieces per split task (the chunk grows beyond that)
+
+// Steps per piece when a task of `steps` loop steps is split: as many pieces as
+// max_pieces allows, each of at least split_c steps (host and device schedulers both
+// use this; the chunk travels in the descriptor's iters field of a TF_SPLIT task).
+__host__ __device__ inline int split_chunk_for(int steps, int split_c, int max_pieces) {
+    const int c = split_c > 0 ? split_c : 1;
+    int pieces = (steps + c - 1) / c;
+    if (max_pieces > 0 && pieces > max_pieces) pieces = max_pieces;
+    if (pieces < 1) pieces = 1;
+    return (steps + pieces - 1) / pieces;
+}
 
 struct TaskDesc {
     unsigned long long off;
@@ -125,6 +135,7 @@ struct GCtr {
     unsigned long long truncated;  // children not pushed: deeper than max_depth
     unsigned long long spill_state[4];  // first failed stack write: use, request, launch state, pending
     unsigned long long pushbacks;  // tasks whose rest went back to the stack for lack of room
+    unsigned long long slices;     // tasks whose rest went back to the stack at a time-slice end
 };
 
 struct KParams {
@@ -150,13 +161,14 @@ struct KParams {
     const int32_t* ltn;
     uint32_t* scratch;
     unsigned long long stride;  // uint32 words per workgroup
-    int32_t split_c;            // least branches per piece of a split task
     TaskLog* log;               // task log (null = off)
     unsigned long long log_cap;
     int32_t kloc;               // largest vertex count of one graph (scratch arrays, bitmaps)
     int32_t sum_shift;          // least log2 ids per live-summary bit (HBM-bitmap variant)
     uint32_t max_depth;         // children deeper than this are counted, not pushed (0 = no bound)
     int32_t root_steps;         // > 0: a level-0 task runs only its loop steps s < root_steps
+    int32_t root_first;         // children of level-0 steps s < root_first are counted, not pushed
+    unsigned long long slice_ticks;  // > 0: time slice per launch in realtime-counter ticks (100 MHz)
     unsigned long long maxtot;  // largest task of the batch (~0ull: q->maxtot, device-resident steps)
     const int32_t* gbase;       // batched graphs: first id of each graph (multiples of 32); null = one graph
 };
@@ -256,6 +268,8 @@ struct W {
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
     uint32_t nbranch;             // branches run for the current task / piece
     uint32_t launch_steps;        // stack checks this workgroup passed in this launch
+    unsigned long long t_start;   // thread 0: realtime counter at the launch's start (time slices)
+    bool discard_child;           // the current level-0 branch lies before root_first
     Field pom, poc, pim, pic, com, coc, cim, cic;  // parent and child degrees (4 per id)
     uint32_t *pl2, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
@@ -288,26 +302,49 @@ struct W {
         return v;
     }
 
-    // Visits every entry of the list a[b, e) once, spread over the workgroup. Each thread
-    // issues SCAN_UNR independent loads before it uses any of them, so a long list costs
-    // one memory round trip per SCAN_UNR * NT entries instead of one per NT.
+    // Bitmap word holding id's bit, or 0 when the live-set summary rules the id out (the
+    // HBM-bitmap variant); callers issue a whole batch of these before testing any, so a
+    // batch of list entries costs one memory round trip for its bitmap words.
+    __device__ __forceinline__ uint32_t bword(const uint32_t* bm, int id) const {
+        if (!bm) return 0u;
+        const bool m = !lsum || maybe_live(id);
+        return m ? bm[(id - gb) >> 5] : 0u;
+    }
+    __device__ __forceinline__ static uint32_t bbit(uint32_t word, int id) { return (word >> (id & 31)) & 1u; }
+
+    // Visits every entry of the list a[b, e) once, spread over the workgroup, as
+    // f(id, bit in bmA, bit in bmB) (bmB may be null: bit 0). Each thread issues SCAN_UNR
+    // independent list loads, then the bitmap words of all of them, before it uses any,
+    // so a long list costs one memory round trip per SCAN_UNR * NT entries for the
+    // entries and one for their bitmap words, instead of one per NT entries and per test.
+    // f must not change bmA or bmB.
     template <class F>
-    __device__ __forceinline__ void wg_scan(const int32_t* __restrict__ a, unsigned long long b,
-                                            unsigned long long e, F&& f) const {
+    __device__ __forceinline__ void wg_scan_bits(const int32_t* __restrict__ a, unsigned long long b,
+                                                 unsigned long long e, const uint32_t* bmA, const uint32_t* bmB,
+                                                 F&& f) const {
         unsigned long long i = b + (unsigned long long)tid;
         for (; i + (SCAN_UNR - 1) * NT < e; i += SCAN_UNR * NT) {
             int v[SCAN_UNR];
+            uint32_t wa[SCAN_UNR], wb[SCAN_UNR];
 #pragma unroll
             for (int u = 0; u < SCAN_UNR; ++u) v[u] = a[i + (unsigned long long)u * NT];
 #pragma unroll
-            for (int u = 0; u < SCAN_UNR; ++u) f(v[u]);
+            for (int u = 0; u < SCAN_UNR; ++u) { wa[u] = bword(bmA, v[u]); wb[u] = bword(bmB, v[u]); }
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u) f(v[u], bbit(wa[u], v[u]), bbit(wb[u], v[u]));
         }
         int v[SCAN_UNR];
+        uint32_t wa[SCAN_UNR], wb[SCAN_UNR];
 #pragma unroll
         for (int u = 0; u < SCAN_UNR; ++u) v[u] = i + (unsigned long long)u * NT < e ? a[i + (unsigned long long)u * NT] : -1;
 #pragma unroll
+        for (int u = 0; u < SCAN_UNR; ++u) {
+            wa[u] = v[u] >= 0 ? bword(bmA, v[u]) : 0u;
+            wb[u] = v[u] >= 0 ? bword(bmB, v[u]) : 0u;
+        }
+#pragma unroll
         for (int u = 0; u < SCAN_UNR; ++u)
-            if (v[u] >= 0) f(v[u]);
+            if (v[u] >= 0) f(v[u], bbit(wa[u], v[u]), bbit(wb[u], v[u]));
     }
 
     // Visits a[b, e) with the calling thread alone, 8 loads in flight.
@@ -322,6 +359,26 @@ struct W {
             for (int u = 0; u < 8; ++u) f(v[u]);
         }
         for (; b < e; ++b) f(a[b]);
+    }
+    // As thread_scan, with the bits of two bitmaps fetched per batch of 8 (as wg_scan_bits).
+    template <class F>
+    __device__ __forceinline__ void thread_scan_bits(const int32_t* __restrict__ a, unsigned long long b,
+                                                     unsigned long long e, const uint32_t* bmA, const uint32_t* bmB,
+                                                     F&& f) const {
+        for (; b + 8 <= e; b += 8) {
+            int v[8];
+            uint32_t wa[8], wb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = a[b + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) { wa[u] = bword(bmA, v[u]); wb[u] = bword(bmB, v[u]); }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) f(v[u], bbit(wa[u], v[u]), bbit(wb[u], v[u]));
+        }
+        for (; b < e; ++b) {
+            const int w = a[b];
+            f(w, bbit(bword(bmA, w), w), bbit(bword(bmB, w), w));
+        }
     }
 
     // ---------------------------------------------------------------- predicates
@@ -512,15 +569,18 @@ struct W {
     }
 
     // Number of entries of nbr[b, e) whose id is a current candidate (ccand bit), for one
-    // thread, with four independent loads in flight per step.
+    // thread: 8 list loads, then their 8 bitmap words, in flight per step.
     __device__ __forceinline__ uint32_t count_cand(const int32_t* nbr, unsigned long long b, unsigned long long e) const {
         uint32_t c = 0;
         for (; b + 8 <= e; b += 8) {
             int a[8];
+            uint32_t w[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) a[u] = nbr[b + u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) c += (uint32_t)bt(ccand, a[u]);
+            for (int u = 0; u < 8; ++u) w[u] = bword(ccand, a[u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c += bbit(w[u], a[u]);
         }
         for (; b < e; ++b) c += (uint32_t)bt(ccand, nbr[b]);
         return c;
@@ -531,8 +591,8 @@ struct W {
         unsigned long long i = b + lane;
         for (; i + 3 * 64 < e; i += 4 * 64) {
             const int a0 = nbr[i], a1 = nbr[i + 64], a2 = nbr[i + 128], a3 = nbr[i + 192];
-            c += (uint32_t)bt(ccand, a0) + (uint32_t)bt(ccand, a1) + (uint32_t)bt(ccand, a2) +
-                 (uint32_t)bt(ccand, a3);
+            const uint32_t w0 = bword(ccand, a0), w1 = bword(ccand, a1), w2 = bword(ccand, a2), w3 = bword(ccand, a3);
+            c += bbit(w0, a0) + bbit(w1, a1) + bbit(w2, a2) + bbit(w3, a3);
         }
         for (; i < e; i += 64) c += (uint32_t)bt(ccand, nbr[i]);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
@@ -575,11 +635,11 @@ struct W {
         const unsigned long long *oo = P->oo, *io = P->io;
         for (int i = tid; i < nd; i += NT) {
             int u = D[i];
-            thread_scan(P->on, oo[u], oo[u + 1], [&](int w) {
-                if (bt(cmem, w) || bt(ccand, w)) atomicSub(&cic[w], 1u);
+            thread_scan_bits(P->on, oo[u], oo[u + 1], cmem, ccand, [&](int w, uint32_t m, uint32_t c) {
+                if (m | c) atomicSub(&cic[w], 1u);
             });
-            thread_scan(P->in, io[u], io[u + 1], [&](int w) {
-                if (bt(cmem, w) || bt(ccand, w)) atomicSub(&coc[w], 1u);
+            thread_scan_bits(P->in, io[u], io[u + 1], cmem, ccand, [&](int w, uint32_t m, uint32_t c) {
+                if (m | c) atomicSub(&coc[w], 1u);
             });
             bytes += 4ull * ((oo[u + 1] - oo[u]) + (io[u + 1] - io[u])) + 32;
         }
@@ -676,6 +736,10 @@ struct W {
     // and pushes the child onto the task stack.
     __device__ __forceinline__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
         const int N = cm + nr;
+        if (discard_child) {  // a level-0 branch before the bounded unit's window
+            if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
+            return -2;
+        }
         if (P->max_depth && cur_depth + 1u > P->max_depth) {  // depth-bounded enumeration
             if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
             return -1;
@@ -832,7 +896,9 @@ struct W {
     // pop leaves that room, so a workgroup's first check in a launch passes unless other
     // workgroups already wrote children (no launch can stall). Its later checks also keep
     // one task of the largest size (kloc) free, so that a launch leaves room for the next.
-    __device__ __forceinline__ bool stack_low() {
+    // Returns 0 (go on), 1 (stack low: defer the rest) or 2 (the launch's time slice is
+    // over: push the rest back onto the stack; never at a workgroup's first check).
+    __device__ __forceinline__ int stack_low() {
         unsigned long long low = 0;
         if (tid == 0) {
             const unsigned long long a = __atomic_load_n(&P->q->used, __ATOMIC_RELAXED);
@@ -843,8 +909,11 @@ struct W {
             const unsigned long long margin = launch_steps ? (unsigned long long)P->kloc : 0ull;
             low = used_t + wg + (pend >> CNT_SHIFT) + (launch_steps ? 1 : 0) > P->sp_cap_t ||
                   used_v + wg * mt + (pend & SIZE_MASK) + margin > P->sp_cap_v;
+            if (!low && P->slice_ticks && launch_steps &&
+                __builtin_amdgcn_s_memrealtime() - t_start > P->slice_ticks)
+                low = 2;
         }
-        const bool r = bcast0(low) != 0;
+        const int r = (int)bcast0(low);
         if (!r) ++launch_steps;
         return r;
     }
@@ -895,22 +964,22 @@ struct W {
         // member falls short; here the in- and two-hop decrements may also have run, which
         // is unobservable because a failure ends the task.
         int fail = 0, skip = 0;
-        wg_scan(P->on, P->oo[p], P->oo[p + 1], [&](int w) {
-            if (bt(plive, w)) {
+        wg_scan_bits(P->on, P->oo[p], P->oo[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+            if (live) {
                 uint32_t ic = pic[w] - 1u;
                 pic[w] = ic;
-                if (pim[w] + ic < mi) { if (bt(pmem, w)) fail = 1; else if (w == last) skip = 1; }
+                if (pim[w] + ic < mi) { if (memb) fail = 1; else if (w == last) skip = 1; }
             }
         });
-        wg_scan(P->in, P->io[p], P->io[p + 1], [&](int w) {
-            if (bt(plive, w)) {
+        wg_scan_bits(P->in, P->io[p], P->io[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+            if (live) {
                 uint32_t oc = poc[w] - 1u;
                 poc[w] = oc;
-                if (pom[w] + oc < mo) { if (bt(pmem, w)) fail = 1; else if (w == last) skip = 1; }
+                if (pom[w] + oc < mo) { if (memb) fail = 1; else if (w == last) skip = 1; }
             }
         });
-        wg_scan(P->tn, P->to[p], P->to[p + 1], [&](int w) {
-            if (bt(plive, w) && !bt(pmem, w)) pl2[w] -= 1u;
+        wg_scan_bits(P->tn, P->to[p], P->to[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+            if (live && !memb) pl2[w] -= 1u;
         });
         if (tid == 0)
             bytes += 4ull * ((P->oo[p + 1] - P->oo[p]) + (P->io[p + 1] - P->io[p]) + (P->to[p + 1] - P->to[p])) + 48;
@@ -956,8 +1025,8 @@ struct W {
             bs(cmem, x);
         }
         // candidates within two hops of v (the others fail diameter pruning)
-        wg_scan(P->tn, P->to[v], P->to[v + 1], [&](int w) {
-            if (bt(plive, w) && !bt(pmem, w) && w != v) {
+        wg_scan_bits(P->tn, P->to[v], P->to[v + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+            if (live && !memb && w != v) {
                 com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
                 bs(cL, w);
                 lst[atomicAdd(&S->cnt[3], 1)] = w;
@@ -968,17 +1037,17 @@ struct W {
         const int nl = S->cnt[3];
         dirty_l = nl;
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
-        wg_scan(P->on, P->oo[v], P->oo[v + 1], [&](int w) {
-            if (bt(cmem, w) || bt(cL, w)) { cim[w] += 1u; cic[w] -= 1u; }
+        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], cmem, cL, [&](int w, uint32_t m, uint32_t l) {
+            if (m | l) { cim[w] += 1u; cic[w] -= 1u; }
         });
-        wg_scan(P->in, P->io[v], P->io[v + 1], [&](int w) {
-            if (bt(cmem, w) || bt(cL, w)) { com[w] += 1u; coc[w] -= 1u; }
+        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cmem, cL, [&](int w, uint32_t m, uint32_t l) {
+            if (m | l) { com[w] += 1u; coc[w] -= 1u; }
         });
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
         const long long t = branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1);
         end_child();
-        return t >= 0;
+        return t >= 0 || t == -2;  // -2: a child outside the bounded unit (still the first child)
     }
 
     __device__ __forceinline__ void begin_child(int cm) {
@@ -1080,8 +1149,10 @@ struct W {
         tick(7);
         if (s == 0 || nr == 0) return -1;
         long long t = write_child(cm, R, nr, child_mode);
-        if (split_index >= 0 && t >= 0 && tid == 0)
-            atomicMin(&P->q->root_first, ((unsigned long long)split_index << 32) | (unsigned long long)t);
+        // the first child of the level-0 task (a discarded one is marked 0xFFFFFFFF)
+        if (split_index >= 0 && (t >= 0 || t == -2) && tid == 0)
+            atomicMin(&P->q->root_first,
+                      ((unsigned long long)split_index << 32) | (t >= 0 ? (unsigned long long)t : 0xFFFFFFFFull));
         tick(6);
         return t;
     }
@@ -1095,14 +1166,14 @@ struct W {
         for (int a = 0; a < cm; ++a) {
             int x = mem[a];
             if (com[x] + coc[x] == tho && coc[x] > 0u) {
-                wg_scan(P->on, P->oo[x], P->oo[x + 1], [&](int w) {
-                    if (bt(ccand, w)) bs(ccrit, w);
+                wg_scan_bits(P->on, P->oo[x], P->oo[x + 1], ccand, nullptr, [&](int w, uint32_t c, uint32_t) {
+                    if (c) bs(ccrit, w);
                 });
                 if (tid == 0) bytes += 4ull * (P->oo[x + 1] - P->oo[x]) + 16;
             }
             if (cim[x] + cic[x] == thi && cic[x] > 0u) {
-                wg_scan(P->in, P->io[x], P->io[x + 1], [&](int w) {
-                    if (bt(ccand, w)) bs(ccrit, w);
+                wg_scan_bits(P->in, P->io[x], P->io[x + 1], ccand, nullptr, [&](int w, uint32_t c, uint32_t) {
+                    if (c) bs(ccrit, w);
                 });
                 if (tid == 0) bytes += 4ull * (P->io[x + 1] - P->io[x]) + 16;
             }
@@ -1131,8 +1202,8 @@ struct W {
         // spread over the workgroup (entries of one list are distinct, so plain adds)
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
-            wg_scan(P->tn, P->to[c], P->to[c + 1], [&](int w) {
-                if (bt(cmem, w) || bt(ccand, w)) adjc[w] += 1u;
+            wg_scan_bits(P->tn, P->to[c], P->to[c + 1], cmem, ccand, [&](int w, uint32_t m, uint32_t k) {
+                if (m | k) adjc[w] += 1u;
             });
             if (tid == 0) bytes += 4ull * (P->to[c + 1] - P->to[c]) + 16;
             __syncthreads();
@@ -1144,11 +1215,11 @@ struct W {
         // the critical vertices join the members (reference src/device_kernels.cu:772-805)
         for (int i = 0; i < nk; ++i) {
             const int c = K[i];
-            wg_scan(P->on, P->oo[c], P->oo[c + 1], [&](int w) {
-                if (bt(cmem, w) || bt(ccand, w)) { cim[w] += 1u; cic[w] -= 1u; }
+            wg_scan_bits(P->on, P->oo[c], P->oo[c + 1], cmem, ccand, [&](int w, uint32_t m, uint32_t k) {
+                if (m | k) { cim[w] += 1u; cic[w] -= 1u; }
             });
-            wg_scan(P->in, P->io[c], P->io[c + 1], [&](int w) {
-                if (bt(cmem, w) || bt(ccand, w)) { com[w] += 1u; coc[w] -= 1u; }
+            wg_scan_bits(P->in, P->io[c], P->io[c + 1], cmem, ccand, [&](int w, uint32_t m, uint32_t k) {
+                if (m | k) { com[w] += 1u; coc[w] -= 1u; }
             });
             if (tid == 0) bytes += 4ull * ((P->oo[c + 1] - P->oo[c]) + (P->io[c + 1] - P->io[c])) + 32;
             __syncthreads();
@@ -1190,9 +1261,10 @@ struct W {
     // src/host_functions.cu:987-1134 (host levels 0 and 1)
     //
     // A task with many branches is split instead: one pass runs only the removal and
-    // lookahead steps, and every split_c steps pushes the state as a piece that runs the
-    // next split_c steps with full branches (its first step removes unless it is the
-    // task's first). The pieces run exactly the branches of the sequential loop, so the
+    // lookahead steps, and every `chunk` steps (the piece length the scheduler chose,
+    // carried in iters) pushes the state as a piece that runs the next `chunk` steps with
+    // full branches (its first step removes unless it is the task's first). The pieces
+    // run exactly the branches of the sequential loop, so the
     // branches, cliques and children are the same; the task is counted once, here.
     __device__ __forceinline__ void run_task(const TaskDesc& d, const dqc::TaskVertex* rv) {
         const int T = d.tot, nm = d.nm, mode = d.mode;
@@ -1216,11 +1288,11 @@ struct W {
         __syncthreads();
         int tot = T, nc = T - nm;
         int expansions = piece ? min(d.iters, nc) : nc;
-        if (mode == MODE_HOST_L0 && P->root_steps > 0) expansions = min(expansions, ncov + P->root_steps);
+        if (mode == MODE_HOST_L0 && P->root_steps > 0) expansions = min(expansions, ncov + P->root_first + P->root_steps);
         // the batch scheduler marks the tasks to split (host side, within a record budget)
-        const int steps = expansions - ncov;
-        const int chunk = max(P->split_c, (steps + MAX_PIECES - 1) / MAX_PIECES);
+        // a task marked TF_SPLIT carries its piece length (steps per piece) in iters
         const bool split = !piece && mode != MODE_HOST_L0 && (d.flags & TF_SPLIT);
+        const int chunk = split ? max(1, d.iters) : 1;
         const bool remove_first = piece && (d.flags & TF_REMOVE_FIRST);
         bool first_host_child = (mode == MODE_HOST_L0);
         for (int j = ncov; j < expansions; ++j) {
@@ -1228,10 +1300,12 @@ struct W {
             // a deferred piece then takes the place of that chunk's piece and no step is
             // covered twice (a check inside a chunk would defer steps its piece already holds).
             const bool chunk_start = (j - ncov) % chunk == 0;
-            if ((!split || chunk_start) && stack_low()) {
-                // backpressure: the rest of the task becomes a deferred piece
-                write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, true);
-                if (tid == 0) atomicAdd(&P->g->pushbacks, 1ull);
+            const int low = (!split || chunk_start) ? stack_low() : 0;
+            if (low) {
+                // backpressure: the rest of the task becomes a deferred piece; at the end of
+                // a time slice it goes back onto the main stack, to run in the next launch
+                write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, low == 1);
+                if (tid == 0) atomicAdd(low == 1 ? &P->g->pushbacks : &P->g->slices, 1ull);
                 break;
             }
             if (split && chunk_start) write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
@@ -1250,7 +1324,9 @@ struct W {
                 break;
             }
             if (split) continue;
+            discard_child = mode == MODE_HOST_L0 && j - ncov < P->root_first;
             bool wrote = branch(tot, nm, mode, first_host_child);
+            discard_child = false;
             ++nbranch;
             tick(0);
             if (wrote) first_host_child = false;
@@ -1328,6 +1404,8 @@ void expand_kernel(KParams P) {
     w.dirty_m = 0;
     w.dirty_l = 0;
     w.launch_steps = 0;
+    w.t_start = __builtin_amdgcn_s_memrealtime();
+    w.discard_child = false;
     if (threadIdx.x == 0) S.acc_bytes = 0;
     __syncthreads();
     while (true) {
@@ -1342,7 +1420,9 @@ void expand_kernel(KParams P) {
             w.cur_depth = d.depth;
             // level-0 branches keep no live bitmap (the live set is an id range)
             if constexpr (BIG) w.fill_live_summary(~0u);
+            w.discard_child = P.split_s[t] < P.root_first;
             w.branch_root(P.split_s[t], d.tot, P.in_vtx + d.off);
+            w.discard_child = false;
             if constexpr (BIG) w.fill_live_summary(0u);
         } else {
             TaskDesc d = P.in_desc[t];
@@ -1444,13 +1524,11 @@ __global__ __launch_bounds__(POP_T) void pop_kernel(PopParams pp) {
         q.maxtot = mtot;
         q.wg_limit = wgl;
         q.pend = 0;
-        // piece-record budget of the split marks, clamped by the free stack records the way
-        // the host pop clamps it (use counted before the pop)
+        // piece-record budget of the split marks: a quarter of the free stack records, as
+        // in the host pop (use counted before the pop)
         {
             const unsigned long long used_v = st_v + df_v;
-            unsigned long long budget = pp.pool_vertices / 16;
-            if (used_v + budget > pp.spill_vertices) budget = pp.spill_vertices > used_v ? (pp.spill_vertices - used_v) / 2 : 0;
-            q.split_budget = budget;
+            q.split_budget = pp.spill_vertices > used_v ? (pp.spill_vertices - used_v) / 4 : 0;
         }
         if (cnt > 0) {
             // first / v0: the batch's lowest descriptor index and record; both regions keep a
@@ -1485,11 +1563,11 @@ __host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0
 // their piece records fits q->split_budget (a prefix rule: the first task that does not
 // fit ends the marking, as in the host pop). A batch without a task of long_task_steps
 // steps keeps stack order and splits nothing. perm[k] = the batch index of the k-th
-// task to run; split[k] = 1 to split it.
+// task to run; split[k] = its piece length when it is to be split, else 0.
 __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
                                                       const float* __restrict__ pred, int learned, int split_b,
-                                                      int split_c, uint32_t* __restrict__ perm,
-                                                      uint8_t* __restrict__ split) {
+                                                      int split_c, int max_pieces, uint32_t* __restrict__ perm,
+                                                      uint32_t* __restrict__ split) {
     __shared__ unsigned hist[64], start[64];
     __shared__ unsigned wcnt[POP_T / 64][64];
     __shared__ unsigned long long scan[POP_T];
@@ -1563,11 +1641,12 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
     for (unsigned long long c0 = 0; c0 < n; c0 += POP_T) {
         const unsigned long long k = c0 + tid;
         unsigned long long need = 0;
+        int chunk = 0;
         if (k < n && split_b > 0) {
             const TaskDesc d = sdesc[first + perm[k]];
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b) {
-                const int chunk = max(split_c, (steps + MAX_PIECES - 1) / MAX_PIECES);
+                chunk = split_chunk_for(steps, split_c, max_pieces);
                 need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
             }
         }
@@ -1579,7 +1658,7 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
             scan[tid] += a;
             __syncthreads();
         }
-        if (k < n) split[k] = need > 0 && carry + scan[tid] <= budget;
+        if (k < n) split[k] = (need > 0 && carry + scan[tid] <= budget) ? (uint32_t)chunk : 0u;
         __syncthreads();
         if (tid == POP_T - 1) carry += scan[tid];
         __syncthreads();
@@ -1590,7 +1669,7 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
 // split marks when perm is given) and the records.
 __global__ void gather_kernel(const TaskDesc* __restrict__ sdesc, const dqc::TaskVertex* __restrict__ svtx,
                               const QCtr* __restrict__ q, const uint32_t* __restrict__ perm,
-                              const uint8_t* __restrict__ split, TaskDesc* __restrict__ bdesc,
+                              const uint32_t* __restrict__ split, TaskDesc* __restrict__ bdesc,
                               dqc::TaskVertex* __restrict__ bvtx) {
     const unsigned long long n = q->ntask;
     if (n == 0) return;
@@ -1602,7 +1681,10 @@ __global__ void gather_kernel(const TaskDesc* __restrict__ sdesc, const dqc::Tas
     for (unsigned long long k = gid; k < n; k += gsz) {
         TaskDesc d = sdesc[first + (perm ? perm[k] : k)];
         d.off -= v0;
-        if (perm && split[k]) d.flags |= TF_SPLIT;
+        if (perm && split[k]) {
+            d.flags |= TF_SPLIT;
+            d.iters = (int32_t)split[k];  // piece length
+        }
         bdesc[k] = d;
     }
     for (unsigned long long i = gid; i < vend - v0; i += gsz) bvtx[i] = svtx[v0 + i];
@@ -1744,7 +1826,7 @@ struct dqc_handle {
     // device-resident steps: one captured HIP graph of resident_steps steps per schedule
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
     uint32_t* d_perm = nullptr;    // batch order (SIZE / LEARNED)
-    uint8_t* d_mark = nullptr;     // split marks in that order
+    uint32_t* d_mark = nullptr;    // split marks in that order (piece length, 0 = whole)
     float* d_model = nullptr;      // cost model (dqc_set_cost_model)
     float* d_cost = nullptr;       // per-task predictions of the current batch
     std::vector<float> hpred;
@@ -2157,6 +2239,9 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
 // records and run all of its branches at once, so the budget bounds the stack growth).
 // The first eligible task that does not fit ends the marking; the device-resident
 // order kernel follows the same prefix rule with the same budget.
+// Most pieces a split task is cut into: dqc_config.split_pieces, or one per workgroup.
+static int max_pieces(const dqc_handle* h) { return h->cfg.split_pieces > 0 ? h->cfg.split_pieces : h->launch_wg; }
+
 static bool batch_has_long_task(const TaskDesc* b, unsigned long long n, int split_b) {
     for (unsigned long long i = 0; i < n; ++i)
         if (task_steps(b[i]) >= long_task_steps(split_b)) return true;
@@ -2230,19 +2315,19 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h->hcost[x] > h->hcost[y]; });
         h->hbatch.resize(cnt);
         const unsigned long long used = h->st_v + h->def_v;
-        unsigned long long budget = h->cfg.pool_vertices / 16;
-        if (used + budget > h->cfg.spill_vertices) budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 2 : 0;
+        unsigned long long budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 4 : 0;
         for (unsigned long long i = 0; i < cnt; ++i) {
             TaskDesc d = b[order[i]];
             d.off -= v0;
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && h->cfg.split_branches > 0 &&
                 steps >= h->cfg.split_branches) {
-                const int chunk = std::max(h->cfg.split_chunk, (steps + MAX_PIECES - 1) / MAX_PIECES);
+                const int chunk = split_chunk_for(steps, h->cfg.split_chunk, max_pieces(h));
                 const unsigned long long need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
                 if (need <= budget) {
                     budget -= need;
                     d.flags |= TF_SPLIT;
+                    d.iters = chunk;  // piece length
                 } else {
                     budget = 0;  // prefix rule: no later task is split in this batch
                 }
@@ -2331,13 +2416,14 @@ static KParams make_params(dqc_handle* h, unsigned long long ntask) {
     P.lto = h->d_lto; P.ltn = h->d_ltn;
     P.scratch = h->d_scratch;
     P.stride = h->stride;
-    P.split_c = h->cfg.split_chunk;
     P.log = h->d_log;
     P.log_cap = h->log_cap;
     P.kloc = h->kloc;
     P.gbase = h->d_gbase;
     P.max_depth = (uint32_t)std::max(0, h->cfg.max_depth);
     P.root_steps = std::max(0, h->cfg.root_branches);
+    P.root_first = std::max(0, h->cfg.root_first);
+    P.slice_ticks = h->cfg.launch_ms > 0 ? (unsigned long long)h->cfg.launch_ms * 100000ull : 0ull;  // 100 MHz
     P.maxtot = std::max<unsigned long long>(h->batch_maxtot, 1);
     return P;
 }
@@ -2351,6 +2437,7 @@ static void fold_stats(dqc_handle* h, const GCtr& g) {
     h->st.slow_sorts = (int64_t)g.slow_sorts;
     h->st.truncated_children = (int64_t)g.truncated;
     h->st.stack_pushbacks = (int64_t)g.pushbacks;
+    h->st.time_slices = (int64_t)g.slices;
     h->st.peak_stack_records = (int64_t)h->peak_v;
     h->st.peak_stack_tasks = (int64_t)h->peak_t;
     for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
@@ -2420,7 +2507,8 @@ static dqc_status split_root(dqc_handle* h) {
     if (!hip_ok(h, hipMemcpy(&d, h->d_bdesc, sizeof d, hipMemcpyDeviceToHost), "root read")) return DQC_E_HIP;
     const int k = d.tot, ncov = d.ncov;
     // a bounded unit of work: only the first root_branches loop steps (dqc_config)
-    const int S = h->cfg.root_branches > 0 ? std::min(k - ncov, h->cfg.root_branches) : k - ncov;
+    const int S = h->cfg.root_branches > 0 ? std::min(k - ncov, std::max(0, h->cfg.root_first) + h->cfg.root_branches)
+                                           : k - ncov;
     KParams P = make_params(h, 0);
     if (S > 0) {
         root_flags_kernel<<<(S + 255) / 256, 256, 0, h->stream>>>(P, h->d_bvtx + d.off, k, S, h->d_flags);
@@ -2455,7 +2543,8 @@ static dqc_status split_root(dqc_handle* h) {
     P.split_s = h->d_split;
     rc = launch(h, P, work.size());
     if (rc != DQC_OK) return rc;
-    if (h->last_root_first != ~0ull) {
+    // the first child keeps the host level-1 semantics, unless it lies before the window
+    if (h->last_root_first != ~0ull && (h->last_root_first & 0xFFFFFFFFull) != 0xFFFFFFFFull) {
         const unsigned long long t = h->last_root_first & 0xFFFFFFFFull;
         const int32_t mode = MODE_HOST_L1;
         if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &mode, sizeof mode,
@@ -2472,14 +2561,15 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
                  h->cfg.spill_vertices, (unsigned long long)h->launch_wg};
     pop_kernel<<<1, POP_T, 0, h->stream>>>(pp);
     const uint32_t* perm = nullptr;
-    const uint8_t* mark = nullptr;
+    const uint32_t* mark = nullptr;
     if (sched != DQC_SCHED_FIFO) {
         if (sched == DQC_SCHED_LEARNED) {
             const int blocks = (int)std::min<unsigned long long>(1024, (h->cfg.pool_tasks + 255) / 256);
             desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc, 0, h->d_cost, h->d_q);
         }
         order_kernel<<<1, POP_T, 0, h->stream>>>(h->d_sdesc, h->d_q, h->d_cost, sched == DQC_SCHED_LEARNED,
-                                                 h->cfg.split_branches, h->cfg.split_chunk, h->d_perm, h->d_mark);
+                                                 h->cfg.split_branches, h->cfg.split_chunk, max_pieces(h), h->d_perm,
+                                                 h->d_mark);
         perm = h->d_perm;
         mark = h->d_mark;
     }
@@ -2655,6 +2745,12 @@ dqc_status dqc_fetch_batch_cliques(dqc_handle* h, int64_t* count, int64_t* total
         for (int32_t i = 0; i < len; ++i) ids[p + i] -= h->gbase[g];
         p += (size_t)len;
     }
+    return DQC_OK;
+}
+
+dqc_status dqc_batch_size(const dqc_handle* h, int32_t* ngraphs) {
+    if (!h || !ngraphs) return DQC_E_ARG;
+    *ngraphs = h->ngraphs;
     return DQC_OK;
 }
 

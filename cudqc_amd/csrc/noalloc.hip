@@ -138,7 +138,6 @@ extern "C" dqc_status dqc_expand_noalloc(const dqc_device_graph* graph, const dq
         P.ltn = nullptr;
         P.scratch = base + NOALLOC_CTR_WORDS;
         P.stride = NOALLOC_STRIDE;
-        P.split_c = 32;
         P.log = nullptr;
         P.log_cap = 0;
         P.kloc = std::max(graph->n, 1);

@@ -380,6 +380,42 @@ dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int3
     return DQC_OK;
 }
 
+dqc_status dqc_finish_batch(dqc_handle* h, const char* const* max_paths, int64_t* max_cliques, dqc_stats* st) {
+    if (!h) return DQC_E_ARG;
+    int64_t count = 0, total = 0;
+    dqc_status s = dqc_fetch_batch_cliques(h, &count, &total, nullptr, nullptr, nullptr);
+    if (s != DQC_OK) return s;
+    std::vector<int32_t> gid((size_t)count), sizes((size_t)count), ids((size_t)total);
+    s = dqc_fetch_batch_cliques(h, &count, &total, gid.data(), sizes.data(), ids.data());
+    if (s != DQC_OK) return s;
+    dqc_stats stats{};
+    dqc_get_stats(h, &stats);
+    int32_t ngraphs = 0;
+    s = dqc_batch_size(h, &ngraphs);
+    if (s != DQC_OK) return s;
+    std::vector<std::vector<std::vector<int32_t>>> sets((size_t)ngraphs);
+    size_t p = 0;
+    for (int64_t c = 0; c < count; ++c) {
+        if (gid[c] < 0 || gid[c] >= ngraphs) return DQC_E_ARG;
+        sets[gid[c]].emplace_back(ids.begin() + p, ids.begin() + p + sizes[c]);
+        p += (size_t)sizes[c];
+    }
+    auto tr = std::chrono::steady_clock::now();
+    std::vector<std::vector<std::vector<int32_t>>> mx((size_t)ngraphs);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < ngraphs; ++i) mx[i] = dqc::maximal_sets(sets[i]);
+    stats.rmnonmax_ms = ms_since(tr);
+    int64_t all = 0;
+    for (int32_t i = 0; i < ngraphs; ++i) {
+        if (max_paths && max_paths[i] && !mx[i].empty() && !write_sets(max_paths[i], mx[i])) return DQC_E_IO;
+        if (max_cliques) max_cliques[i] = (int64_t)mx[i].size();
+        all += (int64_t)mx[i].size();
+    }
+    stats.maximal_cliques = all;
+    if (st) *st = stats;
+    return DQC_OK;
+}
+
 dqc_status dqc_search_batch(const char* const* graph_paths, int32_t ngraphs, double out_gamma, double in_gamma,
                             int32_t min_size, const char* const* max_paths, const dqc_config* cfg,
                             int64_t* max_cliques, dqc_stats* st) {
@@ -398,32 +434,9 @@ dqc_status dqc_search_batch(const char* const* graph_paths, int32_t ngraphs, dou
     const double prep_ms = ms_since(tp);
     s = dqc_run(h, DQC_SCHED_DEFAULT);
     if (s != DQC_OK) return done(s);
-    int64_t count = 0, total = 0;
-    s = dqc_fetch_batch_cliques(h, &count, &total, nullptr, nullptr, nullptr);
-    if (s != DQC_OK) return done(s);
-    std::vector<int32_t> gid((size_t)count), sizes((size_t)count), ids((size_t)total);
-    s = dqc_fetch_batch_cliques(h, &count, &total, gid.data(), sizes.data(), ids.data());
-    if (s != DQC_OK) return done(s);
     dqc_stats stats{};
-    dqc_get_stats(h, &stats);
-    std::vector<std::vector<std::vector<int32_t>>> sets((size_t)ngraphs);
-    size_t p = 0;
-    for (int64_t c = 0; c < count; ++c) {
-        sets[gid[c]].emplace_back(ids.begin() + p, ids.begin() + p + sizes[c]);
-        p += (size_t)sizes[c];
-    }
-    int64_t all = 0;
-    auto tr = std::chrono::steady_clock::now();
-    std::vector<std::vector<std::vector<int32_t>>> mx((size_t)ngraphs);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int32_t i = 0; i < ngraphs; ++i) mx[i] = dqc::maximal_sets(sets[i]);
-    stats.rmnonmax_ms = ms_since(tr);
-    for (int32_t i = 0; i < ngraphs; ++i) {
-        if (max_paths && max_paths[i] && !mx[i].empty() && !write_sets(max_paths[i], mx[i])) return done(DQC_E_IO);
-        if (max_cliques) max_cliques[i] = (int64_t)mx[i].size();
-        all += (int64_t)mx[i].size();
-    }
-    stats.maximal_cliques = all;
+    s = dqc_finish_batch(h, max_paths, max_cliques, &stats);
+    if (s != DQC_OK) return done(s);
     stats.preprocess_ms = prep_ms;
     if (st) *st = stats;
     return done(DQC_OK);

@@ -53,17 +53,28 @@ typedef struct {
                                   device, resident_steps per replay of one captured HIP
                                   graph, with no host round trip between them;
                                   0 = host-driven steps                               */
-    int32_t root_branches;     /* > 0: only the level-0 loop steps s < root_branches of
-                                  a single graph run (with max_depth, a fixed bounded
-                                  unit of work: the first level-0 branches and their
-                                  subtrees down to that depth); 0 = all              */
+    int32_t root_branches;     /* > 0: only the level-0 loop steps s < root_first +
+                                  root_branches of a single graph run (with max_depth, a
+                                  fixed bounded unit of work: a window of level-0
+                                  branches and their subtrees down to that depth);
+                                  0 = all                                             */
+    int32_t split_pieces;      /* most pieces one split task is cut into; 0 = as many
+                                  as a launch has workgroups                          */
+    int32_t root_first;        /* the children of level-0 steps s < root_first are
+                                  counted in truncated_children, not expanded (the
+                                  branches still run, so the host level-1 child is the
+                                  reference's); 0 = none                              */
+    int32_t launch_ms;         /* > 0: a task still running after launch_ms in its
+                                  launch pushes the rest of its steps back onto the
+                                  stack as a piece and the launch ends soon after
+                                  (output unchanged); 0 = launches run to completion  */
 } dqc_config;
 
 typedef struct {
     int64_t tasks_expanded;    /* tasks whose branch loop ran                         */
     int64_t branches;          /* add-one-vertex steps                                */
     int64_t raw_cliques;       /* cliques emitted before the maximality filter        */
-    int64_t maximal_cliques;   /* lines of the maximal result (after dqc_finish)      */
+    int64_t maximal_cliques;   /* lines of the maximal result (search / finish calls) */
     int64_t bytes_scanned;     /* CSR + task-record bytes the kernels touched         */
     int64_t launches;          /* expansion kernel launches                           */
     int64_t slow_sorts;        /* children ordered by the fallback sort               */
@@ -79,6 +90,8 @@ typedef struct {
     double preprocess_ms;      /* search entry points: graph load (or adoption), two-hop
                                   lists, level 0 + condense and upload                 */
     double rmnonmax_ms;        /* search entry points: the maximal-set filter         */
+    int64_t time_slices;       /* tasks whose rest went back onto the stack at the end
+                                  of a launch_ms time slice                           */
 } dqc_stats;
 
 /* Receives each maximal set (ascending ids, sets in the DQC-R file's order). */
@@ -287,6 +300,14 @@ void dqc_free(void* p);
  * graph g's condensed size. */
 dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int32_t ngraphs, double out_gamma,
                              double in_gamma, int32_t min_size, int32_t* n_condensed);
+/* Graphs in the handle's id space (1 for a single graph). */
+dqc_status dqc_batch_size(const dqc_handle* h, int32_t* ngraphs);
+/* Result dump of a batched run (the per-graph h_dump_cliques + RemoveNonMax of the
+ * reference's path, src/host_functions.cu:1208-1237 and program.cu:145-170): each
+ * graph's raw cliques through the maximal-set filter; max_paths[g] (NULL entries skip)
+ * gets graph g's maximal file (not created when empty), max_cliques[g] its count;
+ * st (optional) the handle's stats with maximal_cliques and rmnonmax_ms set. */
+dqc_status dqc_finish_batch(dqc_handle* h, const char* const* max_paths, int64_t* max_cliques, dqc_stats* st);
 /* The whole path for a batch of graphs in one handle: max_paths[g] (NULL entries skip)
  * gets graph g's maximal file in RemoveNonMax format, max_cliques[g] its count. */
 dqc_status dqc_search_batch(const char* const* graph_paths, int32_t ngraphs, double out_gamma, double in_gamma,

@@ -144,6 +144,35 @@ class Tasks:
             self.lib.orc_free(x)
         return out, sorted(cl), br.value
 
+    def expand_digest(self, records, nm, ncov=0, mode=2, max_steps=0):
+        """As expand, with each child as (SHA-1 of its records' bytes, nm): records are 6
+        uint32 per vertex, the layout the library's task stack uses."""
+        import hashlib
+        flat = (ctypes.c_uint32 * max(6 * len(records), 1))(*[x for r in records for x in r])
+        kids = ctypes.POINTER(ctypes.c_uint32)()
+        meta = ctypes.POINTER(ctypes.c_int64)()
+        knm = ctypes.POINTER(ctypes.c_int)()
+        clq = ctypes.POINTER(ctypes.c_int)()
+        clen = ctypes.c_int64()
+        br = ctypes.c_int64()
+        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), ctypes.byref(kids),
+                                ctypes.byref(meta), ctypes.byref(knm), ctypes.byref(clq), ctypes.byref(clen),
+                                ctypes.byref(br))
+        total = sum(meta[2 * c + 1] for c in range(n))
+        buf = ctypes.string_at(kids, 24 * total) if total else b""
+        out = []
+        for c in range(n):
+            off, size = meta[2 * c], meta[2 * c + 1]
+            out.append((hashlib.sha1(buf[24 * off:24 * (off + size)]).hexdigest(), knm[c]))
+        cl, p = [], 0
+        while p < clen.value:
+            L = clq[p]
+            cl.append(tuple(sorted(clq[p + 1:p + 1 + L])))
+            p += 1 + L
+        for x in (kids, meta, knm, clq):
+            self.lib.orc_free(x)
+        return out, sorted(cl), br.value
+
     def close(self):
         if self.h:
             self.lib.orc_close(self.h)

@@ -130,3 +130,67 @@ def test_hbm_bitmaps_same_result(dqc, manifest, tmp_path, monkeypatch, summary_s
                 or st.raw_cliques != c["raw_cliques"]):
             bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"]))
     assert not bad, bad
+
+
+DEEP = ["workload3_g0", "workload3_g2", "workload3_g8", "rand017", "rand069", "rand088", "bigk20000"]
+
+
+@pytest.mark.parametrize("sched", ["SCHED_FIFO", "SCHED_SIZE"])
+def test_time_slices_same_result(dqc, manifest, tmp_path, sched):
+    """Launches cut into 1 ms time slices: a task still running at the end of its slice
+    pushes the rest of its steps back onto the stack as a piece. Same bytes and counts,
+    and the slicing happened."""
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.schedule = getattr(dqc, sched)
+    cfg.launch_ms = 1
+    slices = 0
+    for c in manifest["cases"]:
+        if c["name"] not in DEEP:
+            continue
+        cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        assert got == exp and cnt == c["maximal"], c["name"]
+        assert (st.tasks_expanded, st.branches, st.raw_cliques) == (c["tasks"], c["branches"], c["raw_cliques"]), c["name"]
+        slices += st.time_slices
+    print("time slices", slices, flush=True)
+    assert slices > 0
+
+
+def level0_children(dqc, path, c, first, count, serial, monkeypatch):
+    if serial:
+        monkeypatch.setenv("DQC_SERIAL_ROOT", "1")
+    else:
+        monkeypatch.delenv("DQC_SERIAL_ROOT", raising=False)
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.root_first, cfg.root_branches = first, count
+    with dqc.Engine(cfg) as eng:
+        eng.prepare(path, c["out_gamma"], c["in_gamma"], c["min_size"])
+        eng.step(1)
+        kids = eng.tasks()
+        st = eng.stats()
+    return kids, st
+
+
+def test_level0_window(dqc, manifest, monkeypatch):
+    """A window [a, a + b) of level-0 steps (root_first / root_branches): its children are
+    those of the prefix [0, a + b) minus those of the prefix [0, a); the others are
+    counted as truncated; the first child keeps host level-1 semantics only if it lies in
+    the window. Branch-parallel and serial level 0 agree."""
+    from collections import Counter
+    for name, a, b in (("workload3_g2", 3, 25), ("workload3_g0", 1, 30), ("rand069", 0, 12), ("bigk20000", 40, 60)):
+        c = next(x for x in manifest["cases"] if x["name"] == name)
+        path = os.path.join(GOLDEN, c["graph"])
+        full, _ = level0_children(dqc, path, c, 0, a + b, False, monkeypatch)
+        head, _ = level0_children(dqc, path, c, 0, a, False, monkeypatch) if a else ([], None)
+        expect = Counter((tuple(r), nm) for r, nm, _ in full) - Counter((tuple(r), nm) for r, nm, _ in head)
+        for serial in (False, True):
+            win, st = level0_children(dqc, path, c, a, b, serial, monkeypatch)
+            got = Counter((tuple(r), nm) for r, nm, _ in win)
+            assert got == expect, (name, serial)
+            assert st.truncated_children == len(head), (name, serial)
+            modes = sorted(m for _, _, m in win)
+            first_in_window = not head and bool(full)
+            assert modes.count(1) == (1 if first_in_window and win else 0), (name, serial, modes)

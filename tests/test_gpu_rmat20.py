@@ -1,0 +1,104 @@
+"""GPU parity at full size on the R-MAT 2^20 workload (2^24 arcs + 500 planted sets of
+20-40 vertices at 0.95; gamma 0.9/0.9, min size 18). Level 0 condenses it to 132,279
+vertices, so the HBM-bitmap kernel variant runs.
+
+Its enumeration is far beyond the oracle, so parity is checked task by task, as for the
+R-MAT 2^16 workload: a window of level-0 branches (including the first child, which
+keeps the host level-1 semantics), and a sample of the depth-1 tasks those branches
+produce, each expanded alone by the GPU (cut into pieces, so the whole chip runs it) and
+by the oracle. Every child record (ids, four degrees, lvl2adj, order), member count,
+clique and the branch count must be equal (children of depth-1 tasks are compared by a
+SHA-1 of their record bytes, tests/test_oracle.py pins that digest form).
+"""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+GO, GI, MN = 0.9, 0.9, 18
+STEPS = 40  # level-0 loop steps of the window
+
+
+@pytest.fixture(scope="module")
+def w4(tmp_path_factory):
+    import cudqc_amd as dq
+    path = str(tmp_path_factory.mktemp("w4") / "workload4.txt")
+    print("generating the R-MAT 2^20 workload", flush=True)
+    assert dq.generate(4, path) == 1 << 20
+    return path
+
+
+@pytest.fixture(scope="module")
+def oracle_w4(w4):
+    print("oracle: load, two-hop lists, level 0", flush=True)
+    t = orc.Tasks(w4, GO, GI, MN, threads=min(os.cpu_count() or 1, 16))
+    root, ncov = t.root()
+    print(f"oracle: {len(root)} condensed vertices, {ncov} covered; expanding {STEPS} level-0 steps", flush=True)
+    kids, cliques, branches = t.expand(root, 0, ncov, orc.Tasks.HOST_L0, max_steps=STEPS)
+    yield t, root, ncov, kids, cliques, branches
+    t.close()
+
+
+def test_level0_window_matches_oracle(dqc, w4, oracle_w4):
+    """The first level-0 steps, branch-parallel on the GPU, equal the oracle's sequential
+    loop over the same steps (the bounded unit of work root_branches defines)."""
+    _, root, ncov, okids, ocl, obr = oracle_w4
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.root_branches = STEPS
+    with dqc.Engine(cfg) as eng:
+        k = eng.prepare(w4, GO, GI, MN)
+        assert k == len(root) == 132279
+        assert eng.task_meta()[0][3] == ncov
+        eng.step(1)
+        tasks = eng.tasks()
+        cl = eng.cliques()
+        st = eng.stats()
+    print(f"level 0: {len(okids)} children, {obr} branches, {len(ocl)} cliques", flush=True)
+    assert okids and st.branches == obr
+    assert sorted((tuple(r), nm) for r, nm, _ in tasks) == sorted((tuple(r), nm) for r, nm in okids)
+    first = [t for t in tasks if t[2] == 1]
+    assert len(first) == 1 and first[0][0] == okids[0][0]
+    assert sorted(tuple(sorted(c)) for c in cl) == ocl
+
+
+def test_depth1_tasks_match_oracle(dqc, w4, oracle_w4):
+    """A deterministic sample of the window's depth-1 tasks (the first child, which holds
+    nearly the whole graph, is left to the level-0 test: its 124k branches are beyond the
+    oracle's time here), each split into pieces on the GPU and compared with the oracle's
+    unsplit expansion."""
+    ot, _, _, okids, _, _ = oracle_w4
+    rest = sorted(range(1, len(okids)), key=lambda i: len(okids[i][0]))
+    sample = sorted(set(rest[:6] + rest[len(rest) // 3:len(rest) // 3 + 2] + rest[-2:]))
+    assert len(sample) >= 8, len(okids)
+    print("sample sizes", [len(okids[i][0]) for i in sample], flush=True)
+    with ThreadPoolExecutor(max_workers=len(sample)) as pool:  # the oracle runs meanwhile
+        futs = {i: pool.submit(ot.expand_digest, okids[i][0], okids[i][1], 0, orc.Tasks.DEVICE) for i in sample}
+        cfg = dqc.Config()
+        cfg.host_addfail_check = 1
+        cfg.schedule = dqc.SCHED_SIZE
+        cfg.split_branches, cfg.split_chunk = 64, 8
+        got = {}
+        with dqc.Engine(cfg) as eng:
+            eng.prepare(w4, GO, GI, MN)
+            for i in sample:
+                recs, nm = okids[i]
+                eng.reset()
+                eng.push_task(recs, nm, 0, orc.Tasks.DEVICE)
+                eng.step(1)  # the split pass: pieces only
+                assert eng.stats().tasks_expanded == 1 and eng.stats().branches == 0, i
+                eng.step(1)  # every piece
+                st = eng.stats()
+                got[i] = (sorted((h, m) for h, m, _ in eng.task_digests()),
+                          sorted(tuple(sorted(c)) for c in eng.cliques()), st.branches, st.tasks_expanded)
+                print(f"task {i}: size {len(recs)}, {st.branches} branches, {len(got[i][0])} children", flush=True)
+        for i in sample:
+            kids, cl, br = futs[i].result()
+            assert got[i][3] == 1, i
+            assert got[i][2] == br, i
+            assert got[i][0] == sorted(kids), i
+            assert got[i][1] == cl, i

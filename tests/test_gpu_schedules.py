@@ -101,34 +101,51 @@ def test_learned_gain_depth_bounded_rmat(dqc):
     assert gain >= 1.5, gain
 
 
-def test_learned_gain_on_batch_workload(dqc, tmp_path):
-    """The 256-graph batch, enumerated completely, plain FIFO vs the learned schedule
-    (both device-resident). Identical counts; the timings are printed."""
+def test_batch_workload_all_graphs_bit_exact(dqc, tmp_path):
+    """The 256-graph batch workload enumerated completely in one handle: every graph's
+    maximal file has the SHA-256 the oracle's has (tests/golden/workload3_all.json), and
+    the batch's tree counts are the sums of the per-graph ones, under plain FIFO with
+    host-driven steps and under the learned schedule with device-resident steps (16 per
+    captured HIP graph); each is run twice and the timings are printed."""
+    import hashlib
+    import json
     from cudqc_amd import learn
+    from tests.conftest import GOLDEN as G
+    exp = json.load(open(os.path.join(G, "workload3_all.json")))
     paths = []
-    for g in range(256):
-        p = str(tmp_path / f"w3_{g}.txt")
-        dqc.generate(3, p, g)
+    for e in exp["graphs"]:
+        p = str(tmp_path / f"w3_{e['graph']}.txt")
+        assert dqc.generate(3, p, e["graph"]) == e["n"]
+        assert hashlib.sha256(open(p, "rb").read()).hexdigest() == e["graph_sha256"], e["graph"]
         paths.append(p)
+    totals = tuple(sum(e[k] for e in exp["graphs"]) for k in ("tasks", "branches", "raw_cliques"))
+    model = learn.load_model()
     out = {}
-    for name, sched in (("fifo", dqc.SCHED_FIFO), ("learned", dqc.SCHED_LEARNED), ("fifo2", dqc.SCHED_FIFO),
-                        ("learned2", dqc.SCHED_LEARNED)):
-        cfg = config(dqc, sched, resident=16)
+    for name, sched, resident in (("fifo", dqc.SCHED_FIFO, 0), ("learned_resident", dqc.SCHED_LEARNED, 16),
+                                  ("fifo2", dqc.SCHED_FIFO, 0), ("learned_resident2", dqc.SCHED_LEARNED, 16)):
+        cfg = config(dqc, sched, resident=resident)
+        mx = [str(tmp_path / f"{name}_{g}.max") for g in range(len(paths))]
         with dqc.Engine(cfg) as eng:
             eng.prepare_batch(paths, 0.85, 0.85, 10)
             if sched == dqc.SCHED_LEARNED:
-                eng.set_cost_model(learn.load_model())
+                eng.set_cost_model(model)
             t = time.perf_counter()
             eng.run()
             wall = time.perf_counter() - t
-            st = eng.stats()
-            out[name] = (wall, st.expand_kernel_ms, st.tasks_expanded, st.branches, st.raw_cliques)
+            counts, st = eng.finish_batch(mx)
+        out[name] = wall
         print(f"{name}: wall {wall:.3f}s kernel {st.expand_kernel_ms:.0f}ms tasks {st.tasks_expanded} "
-              f"branches {st.branches} raw {st.raw_cliques}", flush=True)
-    assert len({v[2:] for v in out.values()}) == 1
-    fifo = min(out["fifo"][0], out["fifo2"][0])
-    learned = min(out["learned"][0], out["learned2"][0])
-    print(f"learned gain over FIFO (wall): {fifo / learned:.3f}x", flush=True)
+              f"maximal {st.maximal_cliques} ({st.tasks_expanded / wall:.0f} tasks/s)", flush=True)
+        assert (st.tasks_expanded, st.branches, st.raw_cliques) == totals, name
+        bad = []
+        for e, m, n in zip(exp["graphs"], mx, counts):
+            got = hashlib.sha256(open(m, "rb").read()).hexdigest() if os.path.exists(m) else ""
+            if got != e["sha256"] or n != e["maximal"]:
+                bad.append(e["graph"])
+        assert not bad, (name, bad)
+    fifo = min(out["fifo"], out["fifo2"])
+    learned = min(out["learned_resident"], out["learned_resident2"])
+    print(f"learned (resident) over FIFO (wall): {fifo / learned:.3f}x", flush=True)
 
 
 # golden cases with the deepest task stacks (most tasks; peak stack well above one task)

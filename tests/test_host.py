@@ -169,3 +169,16 @@ def test_search_graph_checks_its_input(dqc):
             dqc.search_graph(o, i, None, None, None, 0.9, 0.9, 2)
     sets, st = dqc.search_graph(out, inn, None, None, None, 0.9, 0.9, 3)
     assert sets == [] and st.n_condensed == 0 and st.maximal_cliques == 0
+
+
+def test_generator_workload3_all_graphs_pinned(dqc, tmp_path):
+    """The batching workload's 256 graphs: the generator's files have the SHA-256 the
+    per-graph expectations were made from."""
+    import json
+    from tests.conftest import GOLDEN
+    exp = json.load(open(os.path.join(GOLDEN, "workload3_all.json")))
+    assert len(exp["graphs"]) == 256
+    for e in exp["graphs"][::5]:
+        p = str(tmp_path / f"g{e['graph']}.txt")
+        assert dqc.generate(3, p, e["graph"]) == e["n"]
+        assert hashlib.sha256(open(p, "rb").read()).hexdigest() == e["graph_sha256"], e["graph"]

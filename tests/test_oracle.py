@@ -234,3 +234,26 @@ def test_unequal_gammas_sound_and_direction_aware(oracle_lib, tmp_path, seed):
         assert not any(set(a) < set(b) for a in got for b in got), t
         nonempty += bool(got)
     assert nonempty > 20
+
+
+def test_expand_digest_matches_expand(oracle_lib, manifest):
+    """The digest form of a task expansion (used for large tasks) equals the SHA-1 of the
+    full form's record bytes (6 little-endian uint32 per vertex), child by child."""
+    import hashlib
+    import struct
+    from tests.conftest import GOLDEN
+    c = next(x for x in manifest["cases"] if x["name"] == "workload3_g2")
+    t = oracle_lib.Tasks(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"], threads=1)
+    try:
+        root, ncov = t.root()
+        kids, cl, br = t.expand(root, 0, ncov, oracle_lib.Tasks.HOST_L0)
+        dk, dcl, dbr = t.expand_digest(root, 0, ncov, oracle_lib.Tasks.HOST_L0)
+        assert (cl, br) == (dcl, dbr) and len(kids) == len(dk) > 10
+        for (recs, nm), (h, dnm) in zip(kids, dk):
+            assert nm == dnm
+            assert hashlib.sha1(b"".join(struct.pack("<6I", *r) for r in recs)).hexdigest() == h
+        sub, _, _ = t.expand(kids[1][0], kids[1][1], 0, 2, max_steps=3)
+        full, _, _ = t.expand(kids[1][0], kids[1][1], 0, 2)
+        assert len(sub) <= len(full)
+    finally:
+        t.close()
