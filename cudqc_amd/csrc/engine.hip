@@ -71,17 +71,6 @@ constexpr int32_t TF_PIECE = 1;         // a piece of a split task: not counted,
 constexpr int32_t TF_REMOVE_FIRST = 2;  // the piece's first step starts with a removal
 constexpr int32_t TF_SPLIT = 4;         // set by the batch scheduler: cut this task into pieces
 
-// Steps per piece when a task of `steps` loop steps is split: as many pieces as
-// max_pieces allows, each of at least split_c steps (host and device schedulers both
-// use this; the chunk travels in the descriptor's iters field of a TF_SPLIT task).
-__host__ __device__ inline int split_chunk_for(int steps, int split_c, int max_pieces) {
-    const int c = split_c > 0 ? split_c : 1;
-    int pieces = (steps + c - 1) / c;
-    if (max_pieces > 0 && pieces > max_pieces) pieces = max_pieces;
-    if (pieces < 1) pieces = 1;
-    return (steps + pieces - 1) / pieces;
-}
-
 struct TaskDesc {
     unsigned long long off;
     int32_t tot, nm, ncov, mode;
@@ -93,6 +82,32 @@ struct TaskDesc {
     uint16_t dmin_o, dmax_o, dmin_i, dmax_i;  // min / max of out / in degree inside the task
 };
 static_assert(sizeof(TaskDesc) == 56, "task descriptor layout");
+
+// A task the schedulers may split: a whole task (not a piece) below level 0 with at
+// least split_b loop steps.
+__host__ __device__ inline bool split_eligible(const TaskDesc& d, int split_b) {
+    const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
+    return split_b > 0 && !(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b;
+}
+// Most pieces per split task in a batch with n_split eligible tasks: enough for about
+// four pieces per workgroup over the batch (one long task gets one piece per workgroup,
+// many share them), and at most max_pieces.
+__host__ __device__ inline int batch_piece_cap(int max_pieces, int workgroups, unsigned long long n_split) {
+    if (n_split == 0) return max_pieces;
+    const unsigned long long want = (4ull * (unsigned long long)workgroups + n_split - 1) / n_split;
+    const unsigned long long cap = want < 2 ? 2 : want;
+    return (int)(cap < (unsigned long long)max_pieces ? cap : (unsigned long long)max_pieces);
+}
+// Steps per piece when a task of `steps` loop steps is split: as many pieces as
+// max_pieces allows, each of at least split_c steps (host and device schedulers both
+// use this; the chunk travels in the descriptor's iters field of a TF_SPLIT task).
+__host__ __device__ inline int split_chunk_for(int steps, int split_c, int max_pieces) {
+    const int c = split_c > 0 ? split_c : 1;
+    int pieces = (steps + c - 1) / c;
+    if (max_pieces > 0 && pieces > max_pieces) pieces = max_pieces;
+    if (pieces < 1) pieces = 1;
+    return (steps + pieces - 1) / pieces;
+}
 
 // One task-log record per expanded task or piece (the dqc_task_log of the C-ABI).
 struct TaskLog {
@@ -1566,17 +1581,18 @@ __host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0
 // task to run; split[k] = its piece length when it is to be split, else 0.
 __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
                                                       const float* __restrict__ pred, int learned, int split_b,
-                                                      int split_c, int max_pieces, uint32_t* __restrict__ perm,
-                                                      uint32_t* __restrict__ split) {
+                                                      int split_c, int max_pieces, int workgroups,
+                                                      uint32_t* __restrict__ perm, uint32_t* __restrict__ split) {
     __shared__ unsigned hist[64], start[64];
     __shared__ unsigned wcnt[POP_T / 64][64];
     __shared__ unsigned long long scan[POP_T];
     __shared__ unsigned long long carry;
     __shared__ int any_long;
+    __shared__ unsigned n_split;
     const unsigned long long n = q->ntask, first = q->first, budget = q->split_budget;
     const int tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
-    if (tid == 0) { carry = 0; any_long = 0; }
+    if (tid == 0) { carry = 0; any_long = 0; n_split = 0; }
     __syncthreads();
     auto bucket = [&](unsigned long long i) {
         const TaskDesc d = sdesc[first + i];
@@ -1593,9 +1609,12 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
     };
     for (unsigned long long i = tid; i < n; i += POP_T) {
         atomicAdd(&hist[bucket(i)], 1u);
-        if (task_steps(sdesc[first + i]) >= long_task_steps(split_b)) any_long = 1;
+        const TaskDesc d = sdesc[first + i];
+        if (task_steps(d) >= long_task_steps(split_b)) any_long = 1;
+        if (split_eligible(d, split_b)) atomicAdd(&n_split, 1u);
     }
     __syncthreads();
+    const int piece_cap = batch_piece_cap(max_pieces, workgroups, n_split);
     if (!any_long) {  // no task long enough to split or to unbalance the launch: stack order
         for (unsigned long long i = tid; i < n; i += POP_T) { perm[i] = (uint32_t)i; split[i] = 0; }
         return;
@@ -1645,8 +1664,8 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
         if (k < n && split_b > 0) {
             const TaskDesc d = sdesc[first + perm[k]];
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
-            if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b) {
-                chunk = split_chunk_for(steps, split_c, max_pieces);
+            if (split_eligible(d, split_b)) {
+                chunk = split_chunk_for(steps, split_c, piece_cap);
                 need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
             }
         }
@@ -2316,13 +2335,15 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         h->hbatch.resize(cnt);
         const unsigned long long used = h->st_v + h->def_v;
         unsigned long long budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 4 : 0;
+        unsigned long long n_split = 0;
+        for (unsigned long long i = 0; i < cnt; ++i) n_split += split_eligible(b[i], h->cfg.split_branches);
+        const int piece_cap = batch_piece_cap(max_pieces(h), h->launch_wg, n_split);
         for (unsigned long long i = 0; i < cnt; ++i) {
             TaskDesc d = b[order[i]];
             d.off -= v0;
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
-            if (!(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && h->cfg.split_branches > 0 &&
-                steps >= h->cfg.split_branches) {
-                const int chunk = split_chunk_for(steps, h->cfg.split_chunk, max_pieces(h));
+            if (split_eligible(d, h->cfg.split_branches)) {
+                const int chunk = split_chunk_for(steps, h->cfg.split_chunk, piece_cap);
                 const unsigned long long need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
                 if (need <= budget) {
                     budget -= need;
@@ -2568,8 +2589,8 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
             desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc, 0, h->d_cost, h->d_q);
         }
         order_kernel<<<1, POP_T, 0, h->stream>>>(h->d_sdesc, h->d_q, h->d_cost, sched == DQC_SCHED_LEARNED,
-                                                 h->cfg.split_branches, h->cfg.split_chunk, max_pieces(h), h->d_perm,
-                                                 h->d_mark);
+                                                 h->cfg.split_branches, h->cfg.split_chunk, max_pieces(h), h->launch_wg,
+                                                 h->d_perm, h->d_mark);
         perm = h->d_perm;
         mark = h->d_mark;
     }

@@ -40,9 +40,10 @@ typedef struct {
     int32_t host_addfail_check;/* 1 (default 4-rank union) / 0 (single rank, see
                                   reference src/host_functions.cu:1081)                */
     int32_t split_branches;    /* SIZE / LEARNED schedules: a task with at least this
-                                  many branches may be cut into pieces (at most 64, of
-                                  at least split_chunk branches) that run in parallel;
-                                  0 = 1024, negative = never (output is identical)    */
+                                  many branches may be cut into pieces (of at least
+                                  split_chunk branches; see split_pieces) that run in
+                                  parallel; 0 = 1024, negative = never (output is
+                                  identical)                                          */
     int32_t split_chunk;       /* least branches per piece; 0 = 32                    */
     int32_t schedule;          /* DQC_SCHED_* used by dqc_search_file and by
                                   DQC_SCHED_DEFAULT; 0 = DQC_SCHED_SIZE               */
@@ -59,7 +60,9 @@ typedef struct {
                                   branches and their subtrees down to that depth);
                                   0 = all                                             */
     int32_t split_pieces;      /* most pieces one split task is cut into; 0 = as many
-                                  as a launch has workgroups                          */
+                                  as a launch has workgroups. A batch with several
+                                  tasks to split gives each about 4 x workgroups /
+                                  (their number) pieces, so the batch fills a launch  */
     int32_t root_first;        /* the children of level-0 steps s < root_first are
                                   counted in truncated_children, not expanded (the
                                   branches still run, so the host level-1 child is the
