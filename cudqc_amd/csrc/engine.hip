@@ -751,10 +751,7 @@ struct W {
     // and pushes the child onto the task stack.
     __device__ __forceinline__ long long write_child(int cm, const int32_t* R, int nr, int mode) {
         const int N = cm + nr;
-        if (discard_child) {  // a level-0 branch before the bounded unit's window
-            if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
-            return -2;
-        }
+        if (discard_child) return -2;  // a level-0 branch before the bounded unit's window
         if (P->max_depth && cur_depth + 1u > P->max_depth) {  // depth-bounded enumeration
             if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
             return -1;
@@ -1339,7 +1336,11 @@ struct W {
                 break;
             }
             if (split) continue;
-            discard_child = mode == MODE_HOST_L0 && j - ncov < P->root_first;
+            // before a level-0 window only the first child matters (it takes the host
+            // level-1 role); once it is found the branches there are skipped
+            const bool before_window = mode == MODE_HOST_L0 && j - ncov < P->root_first;
+            if (before_window && !first_host_child) continue;
+            discard_child = before_window;
             bool wrote = branch(tot, nm, mode, first_host_child);
             discard_child = false;
             ++nbranch;
@@ -2539,7 +2540,9 @@ static dqc_status split_root(dqc_handle* h) {
     std::vector<uint8_t> flags((size_t)std::max(S, 1));
     if (S > 0 && !hip_ok(h, hipMemcpy(flags.data(), h->d_flags, (size_t)S, hipMemcpyDeviceToHost), "flags read"))
         return DQC_E_HIP;
-    std::vector<int32_t> work;
+    // branches before the window (root_first) and in it
+    std::vector<int32_t> head, work;
+    const int R0 = std::max(0, h->cfg.root_first);
     for (int s = 0; s < S; ++s) {
         const uint8_t f = flags[s];
         if (f & 4) break;          // fewer than min_size vertices remain
@@ -2550,12 +2553,28 @@ static dqc_status split_root(dqc_handle* h) {
             h->host_cliques.push_back(std::move(c));
             break;
         }
-        work.push_back(s);
+        (s < R0 ? head : work).push_back(s);
     }
     h->extra_tasks += 1;  // the level-0 task itself
+    // Before the window only the first child matters: if one exists there, it takes the
+    // host level-1 role and no child of the window does. Those branches run in chunks of
+    // one per workgroup, from the first, until one writes a child (discarded).
+    bool first_before = false;
+    for (size_t c0 = 0; c0 < head.size() && !first_before; c0 += (size_t)h->launch_wg) {
+        const size_t cn = std::min(head.size() - c0, (size_t)h->launch_wg);
+        if (!hip_ok(h, hipMemcpy(h->d_split, head.data() + c0, sizeof(int32_t) * cn, hipMemcpyHostToDevice), "split upload"))
+            return DQC_E_HIP;
+        P.in_ntask = cn;
+        P.split_s = h->d_split;
+        rc = launch(h, P, cn);
+        if (rc != DQC_OK) return rc;
+        first_before = h->last_root_first != ~0ull;
+    }
     if (work.empty()) {
-        h->st.tasks_expanded += 1;
-        h->st.raw_cliques = (int64_t)h->host_cliques.size();
+        if (head.empty()) {  // no launch ran: the counters are the host's
+            h->st.tasks_expanded += 1;
+            h->st.raw_cliques = (int64_t)h->host_cliques.size();
+        }
         return DQC_OK;
     }
     if (!hip_ok(h, hipMemcpy(h->d_split, work.data(), sizeof(int32_t) * work.size(), hipMemcpyHostToDevice), "split upload"))
@@ -2564,8 +2583,8 @@ static dqc_status split_root(dqc_handle* h) {
     P.split_s = h->d_split;
     rc = launch(h, P, work.size());
     if (rc != DQC_OK) return rc;
-    // the first child keeps the host level-1 semantics, unless it lies before the window
-    if (h->last_root_first != ~0ull && (h->last_root_first & 0xFFFFFFFFull) != 0xFFFFFFFFull) {
+    // the first child keeps the host level-1 semantics (unless it lay before the window)
+    if (!first_before && h->last_root_first != ~0ull && (h->last_root_first & 0xFFFFFFFFull) != 0xFFFFFFFFull) {
         const unsigned long long t = h->last_root_first & 0xFFFFFFFFull;
         const int32_t mode = MODE_HOST_L1;
         if (!hip_ok(h, hipMemcpy((char*)(h->d_sdesc + t) + offsetof(TaskDesc, mode), &mode, sizeof mode,

@@ -15,6 +15,10 @@
 
 namespace dqc {
 
+// Graphs up to this many vertices are preprocessed on one thread: on small inputs the
+// cost of waking a thread team exceeds the work (and a capped CPU share makes it worse).
+constexpr int32_t PAR_MIN = 16384;
+
 // ---------------------------------------------------------------------------
 // reference src/common.cpp:15-108. n = number of lines; ids must lie in [0, n);
 // lists sorted, duplicates and self loops removed; m counted after the cleanup.
@@ -43,7 +47,7 @@ int load_text(const std::string& path, Graph& g, std::string& err) {
     // pass 1: parse arcs per line (parallel over lines), collect (u, v) pairs
     std::vector<std::vector<int32_t>> outl(n);
     int bad = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad) if (n > PAR_MIN)
     for (int32_t u = 0; u < n; ++u) {
         size_t p = starts[u];
         size_t e = (u + 1 < n) ? starts[u + 1] : len;
@@ -259,7 +263,7 @@ void twohop_impl(Graph& g, int threads, const std::vector<uint8_t>* src, const s
     // DQC_PROGRESS: one line per tenth of the chunks done (long builds on large graphs)
     const bool progress = std::getenv("DQC_PROGRESS") != nullptr;
     std::atomic<int32_t> chunks_done{0};
-#pragma omp parallel num_threads(threads)
+#pragma omp parallel num_threads(threads) if (n > PAR_MIN)
     {
         // state[x] bit layout, per vertex being processed (reset after each vertex):
         //   bit0 out-neighbour, bit1 in-neighbour, bit2 in the list, bits 8.. epoch stamp
@@ -355,7 +359,7 @@ void twohop_impl(Graph& g, int threads, const std::vector<uint8_t>* src, const s
     g.two.off.assign((size_t)n + 1, 0);
     for (int32_t v = 0; v < n; ++v) g.two.off[v + 1] = g.two.off[v] + cnt[v];
     g.two.nbr.resize(g.two.off[n]);
-#pragma omp parallel for schedule(static) num_threads(threads)
+#pragma omp parallel for schedule(static) num_threads(threads) if (n > PAR_MIN)
     for (int32_t ch = 0; ch < nchunks; ++ch) {
         if (chunk_lists[ch].empty()) continue;
         int32_t v0 = ch * chunk;
@@ -487,7 +491,7 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
     auto restrict_csr = [&](const Csr& src, Csr& dst, std::vector<uint32_t>* len_out) {
         dst.off.assign((size_t)k + 1, 0);
         std::vector<std::vector<int32_t>> rows((size_t)k);
-#pragma omp parallel for schedule(dynamic, 256)
+#pragma omp parallel for schedule(dynamic, 256) if (k > PAR_MIN)
         for (int32_t r = 0; r < k; ++r) {
             int32_t x = order[r];
             std::vector<int32_t>& row = rows[r];
@@ -499,7 +503,7 @@ void level0_condense(Graph& g, const int32_t* md_out, const int32_t* md_in, int 
         }
         for (int32_t r = 0; r < k; ++r) dst.off[r + 1] = dst.off[r] + rows[r].size();
         dst.nbr.resize(dst.off[k]);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (k > PAR_MIN)
         for (int32_t r = 0; r < k; ++r) std::copy(rows[r].begin(), rows[r].end(), dst.nbr.begin() + dst.off[r]);
         if (len_out) {
             len_out->resize((size_t)k);

@@ -176,9 +176,10 @@ def level0_children(dqc, path, c, first, count, serial, monkeypatch):
 
 def test_level0_window(dqc, manifest, monkeypatch):
     """A window [a, a + b) of level-0 steps (root_first / root_branches): its children are
-    those of the prefix [0, a + b) minus those of the prefix [0, a); the others are
-    counted as truncated; the first child keeps host level-1 semantics only if it lies in
-    the window. Branch-parallel and serial level 0 agree."""
+    those of the prefix [0, a + b) minus those of the prefix [0, a); the first child keeps
+    host level-1 semantics only if it lies in the window (the steps before the window run
+    only until they produce a child, which is not expanded). Branch-parallel and serial
+    level 0 agree."""
     from collections import Counter
     for name, a, b in (("workload3_g2", 3, 25), ("workload3_g0", 1, 30), ("rand069", 0, 12), ("bigk20000", 40, 60)):
         c = next(x for x in manifest["cases"] if x["name"] == name)
@@ -190,7 +191,7 @@ def test_level0_window(dqc, manifest, monkeypatch):
             win, st = level0_children(dqc, path, c, a, b, serial, monkeypatch)
             got = Counter((tuple(r), nm) for r, nm, _ in win)
             assert got == expect, (name, serial)
-            assert st.truncated_children == len(head), (name, serial)
+            assert st.truncated_children == 0 and st.tasks_expanded == 1, (name, serial)
             modes = sorted(m for _, _, m in win)
             first_in_window = not head and bool(full)
             assert modes.count(1) == (1 if first_in_window and win else 0), (name, serial, modes)
