@@ -1,21 +1,31 @@
-"""Benchmark: expansion throughput of the MI355X enumerator on the R-MAT 2^16 workload.
+"""Benchmark: maximal directed quasi-clique enumeration on one MI355X.
 
-Workload: directed R-MAT scale 16 (a,b,c,d = .57,.19,.19,.05), 2^20 unique arcs, plus 50
-planted quasi-cliques of 15-30 vertices at arc density 0.95; gamma_out = gamma_in = 0.9,
-min size 12 (synthetic, seeded; generated by cudqc_amd's generator).
+Headline workload: the R-MAT 2^20 graph (scale 20, a,b,c,d = .57/.19/.19/.05, 2^24 unique
+arcs plus 500 planted quasi-cliques of 20-40 vertices at density 0.95; gamma_out =
+gamma_in = 0.9, min size 18; seeded generator). Level 0 condenses it to 132,279 vertices.
 
-A "step" is one expansion batch: the tasks popped from the top of the on-device task
-stack and expanded by one kernel launch (children pushed back onto the stack). The
-full enumeration of this workload runs to tens of millions of tasks and beyond, so the
-benchmark times K steps after W warm-up steps (the level-0 task and its children) and
-reports tasks expanded per second over the timed steps. Inputs (condensed graph, task
-stack) are resident in HBM before timing starts.
+Its full enumeration is far beyond one run, so the unit of work is a fixed, complete slice
+of it: the level-0 loop steps [R0, R0 + B), the depth-1 tasks they produce and the depth-2
+tasks those produce, each expanded in full (children deeper than that are counted, not
+expanded). The level-0 steps before R0 run only as far as needed to find the first child
+(which takes the host level-1 role, so no child of the window does). The unit's tasks,
+branches and raw cliques are the same under every schedule; the benchmark checks that.
 
-Prints one JSON line. --gpus N > 1 is not supported: this path runs on one GPU per process.
+A step is one complete expansion of that unit by the batched + learned form (the LEARNED
+schedule: batches ordered by the cost model, long tasks split into pieces sized to fill
+the chip), from the level-0 task already resident in HBM. W untimed steps, then K timed
+steps; value = tasks expanded per second over the timed steps. The same unit under plain
+FIFO (stack order, no splitting: the plain GPU form) and under SIZE is timed for the
+schedule comparison. Also reported: maximal QCs/s on the unit and on the workloads that
+complete (the 1,000-vertex ER graph and the 256-graph batch); the expansion kernel's
+roofline over the timed steps; the CPU oracle on a bounded sample of the unit's tasks.
+
+Prints one JSON line. --gpus N > 1 exits: this path runs on one GPU per process.
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -23,83 +33,72 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM-side bytes per algorithmic byte of expand_kernel on this workload, from two rocprofv3
-# --pmc passes over this script (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, factors from a 256 MiB
-# copy in the same runs; 1.265 fetched + 0.405 written): profiles/expand_kernel_rmat16_pmc_traffic.md.
-PMC_TRAFFIC_PER_ALGO_BYTE = 1.670
-WORKLOAD = 2
-WORKLOAD_NAME = ("R-MAT 2^16 directed, 2^20 arcs (.57/.19/.19/.05) + 50 planted QCs of 15-30 @0.95; "
-                 "gamma_out=gamma_in=0.9, min_size=12")
+WORKLOAD = 4
+# the bounded unit: level-0 steps [R0, R0 + B), subtrees to depth DEPTH (DQC_UNIT="R0,B,DEPTH"
+# overrides it, for exploring other units; the reported workload names the one used)
+R0, B, DEPTH = (int(x) for x in os.environ.get("DQC_UNIT", "90112,64,2").split(","))
+WORKLOAD_NAME = ("R-MAT 2^20 directed, 2^24 arcs (.57/.19/.19/.05) + 500 planted QCs of 20-40 @0.95; "
+                 "gamma_out=gamma_in=0.9, min_size=18; unit: level-0 steps [%d, %d) and their subtrees to depth %d"
+                 % (R0, R0 + B, DEPTH))
+# HBM traffic of expand_kernel per timed step of this command, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE passes scaled by a copy calibration (profiles/), or None
+PMC_TRAFFIC_PER_STEP = None
 
 
-def cpu_baseline(graph_path, seconds):
-    """Oracle (C++/OpenMP restatement of the reference CPU path) on the same graph, with
-    the expansion capped at `seconds`; tasks expanded per second over the expansion."""
-    from oracle import oracle as orc
-    threads = os.cpu_count() or 1
-    go, gi, mn = 0.9, 0.9, 12
-    r, st = orc.run(graph_path, go, gi, mn, None, None, threads=threads, time_cap_s=seconds)
-    t = max(st.t_expand, 1e-9)
-    cpu = "unknown CPU"
-    try:
-        with open("/proc/cpuinfo") as f:
-            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), cpu)
-    except OSError:
-        pass
-    return {
-        "value": st.tasks / t,
-        "unit": "tasks expanded/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle on {threads} threads of {cpu}, same workload, host levels + first {seconds:.0f}s of the expansion "
-                   f"({st.tasks} tasks, {st.branches} branches, {st.raw_cliques} raw cliques in {t:.1f}s; "
-                   f"preprocessing excluded)"),
-    }
+def unit_config(dq, sched):
+    cfg = dq.Config()
+    cfg.host_addfail_check = 1
+    cfg.schedule = sched
+    cfg.max_depth = DEPTH
+    cfg.root_first = R0
+    cfg.root_branches = B
+    return cfg
 
 
-def schedule_gain(dq, graph, depth=2):
-    """The same workload cut at depth `depth` (a fixed, complete unit of work): tasks
-    expanded per second under plain FIFO, SIZE and the learned schedule, host-driven steps.
-    The counts must agree; the learned form's rate over FIFO's is the schedule gain."""
+def run_unit(eng, root, ncov, sched):
+    """One complete expansion of the unit from the resident level-0 task; returns
+    (wall seconds, stats delta dict)."""
+    eng.reset()
+    eng.push_root(root, ncov)
+    s0 = eng.stats()
+    k0 = s0.expand_kernel_ms
+    t = time.perf_counter()
+    eng.run(sched)
+    wall = time.perf_counter() - t
+    st = eng.stats()
+    return wall, st, st.expand_kernel_ms - k0
+
+
+def maximal_unit(dq, eng, tmp):
+    """The unit's raw cliques through the library's maximal-set filter (RemoveNonMax)."""
+    cl = eng.cliques()
+    raw = os.path.join(tmp, "unit.raw")
+    mx = os.path.join(tmp, "unit.max")
+    with open(raw, "w") as f:
+        for c in cl:
+            f.write(f"{len(c)} " + "".join(f"{x} " for x in c) + "\n")
+    t = time.perf_counter()
+    n = dq.remove_non_max(raw, mx) if cl else 0
+    return n, time.perf_counter() - t
+
+
+def complete_workloads(dq, data_dir):
+    """Maximal QCs/s where the enumeration completes: the 1,000-vertex ER graph (+5 planted
+    sets; gamma 0.8, min 10) through dqc_search_file, and the 256-graph batch (gamma 0.85,
+    min 10) in one handle under the learned schedule with device-resident steps; time =
+    enumeration + maximal-set filter, preprocessing reported apart (median of 3)."""
     from cudqc_amd import learn
-    go, gi, mn = dq.WORKLOADS[WORKLOAD]
-    rows = {}
-    for name, sched in (("fifo", dq.SCHED_FIFO), ("size", dq.SCHED_SIZE), ("learned", dq.SCHED_LEARNED)):
-        cfg = dq.Config()
-        cfg.host_addfail_check = 1  # the default of dqc_open(NULL), as in the timed run above
-        cfg.schedule = sched
-        cfg.max_depth = depth
-        with dq.Engine(cfg) as eng:
-            eng.prepare(graph, go, gi, mn)
-            if sched == dq.SCHED_LEARNED:
-                eng.set_cost_model(learn.load_model())
-            t = time.perf_counter()
-            eng.run()
-            wall = time.perf_counter() - t
-            st = eng.stats()
-        rows[name] = (wall, st.tasks_expanded, st.branches, st.raw_cliques)
-    counts = {v[1:] for v in rows.values()}
-    if len(counts) != 1:
-        raise RuntimeError(f"schedules disagree on the depth-{depth} run: {rows}")
-    tasks = rows["fifo"][1]
-    return {
-        "workload": f"same graph, enumeration cut at depth {depth} (complete, identical counts under every schedule)",
-        "tasks": tasks,
-        "branches": rows["fifo"][2],
-        "fifo_tasks_per_s": tasks / rows["fifo"][0],
-        "size_tasks_per_s": tasks / rows["size"][0],
-        "learned_tasks_per_s": tasks / rows["learned"][0],
-        "learned_over_fifo": rows["fifo"][0] / rows["learned"][0],
-        "size_over_fifo": rows["fifo"][0] / rows["size"][0],
-    }
-
-
-def batch_workload(dq, data_dir):
-    """The 256-graph batch (n log-uniform in [200, 5000], ER background of out-degree 8,
-    1-10 planted sets of 10-25 vertices at 0.92; gamma 0.85, min 10) enumerated completely
-    in one handle: plain FIFO with host-driven steps, and the learned schedule with
-    device-resident steps (16 per captured HIP graph). Counts must agree."""
-    from cudqc_amd import learn
+    out = {}
+    g1 = os.path.join(data_dir, "dqc_workload1.txt")
+    if not os.path.exists(g1):
+        dq.generate(1, g1)
+    runs = []
+    for _ in range(3):
+        cnt, st = dq.search_file(g1, 0.8, 0.8, 10, None, os.path.join(data_dir, "dqc_w1.max"))
+        runs.append((st.expand_wall_ms + st.rmnonmax_ms) / 1e3)
+    t = statistics.median(runs)
+    out["er1000"] = {"maximal": cnt, "tasks": st.tasks_expanded, "seconds": t, "maximal_qcs_per_s": cnt / t,
+                     "tasks_per_s": st.tasks_expanded / t, "preprocess_s": st.preprocess_ms / 1e3}
     d = os.path.join(data_dir, "dqc_batch256")
     os.makedirs(d, exist_ok=True)
     paths = []
@@ -108,30 +107,64 @@ def batch_workload(dq, data_dir):
         if not os.path.exists(p):
             dq.generate(3, p, g)
         paths.append(p)
-    rows = {}
+    runs = []
     for name, sched, resident in (("fifo", dq.SCHED_FIFO, 0), ("learned_resident", dq.SCHED_LEARNED, 16)):
-        cfg = dq.Config()
-        cfg.host_addfail_check = 1
-        cfg.resident_steps = resident
-        with dq.Engine(cfg) as eng:
-            eng.prepare_batch(paths, 0.85, 0.85, 10)
-            if sched == dq.SCHED_LEARNED:
-                eng.set_cost_model(learn.load_model())
-            t = time.perf_counter()
-            eng.run(sched)
-            wall = time.perf_counter() - t
-            st = eng.stats()
-        rows[name] = (wall, st.tasks_expanded, st.branches, st.raw_cliques)
-    if len({v[1:] for v in rows.values()}) != 1:
-        raise RuntimeError(f"batch runs disagree: {rows}")
-    tasks = rows["fifo"][1]
+        times = []
+        for _ in range(3):
+            cfg = dq.Config()
+            cfg.host_addfail_check = 1
+            cfg.resident_steps = resident
+            with dq.Engine(cfg) as eng:
+                eng.prepare_batch(paths, 0.85, 0.85, 10)
+                if sched == dq.SCHED_LEARNED:
+                    eng.set_cost_model(learn.load_model())
+                t = time.perf_counter()
+                eng.run(sched)
+                wall = time.perf_counter() - t
+                counts, st = eng.finish_batch([None] * len(paths))
+            times.append(wall + st.rmnonmax_ms / 1e3)
+            runs.append((name, st.tasks_expanded, st.branches, st.raw_cliques, st.maximal_cliques))
+        t = statistics.median(times)
+        out["batch256_" + name] = {"maximal": st.maximal_cliques, "tasks": st.tasks_expanded, "seconds": t,
+                                   "maximal_qcs_per_s": st.maximal_cliques / t, "tasks_per_s": st.tasks_expanded / t}
+    if len({r[1:] for r in runs}) != 1:
+        raise RuntimeError(f"batch runs disagree: {runs}")
+    out["batch256_learned_over_fifo"] = out["batch256_fifo"]["seconds"] / out["batch256_learned_resident"]["seconds"]
+    return out
+
+
+def cpu_baseline(graph_path, sample_tasks, threads):
+    """The oracle (C++ restatement of the reference's CPU path) expanding a sample of the
+    unit's tasks on the host cores, one task per thread; rate in tasks/s and branches/s.
+    Its own preprocessing (load, two-hop lists, level 0) is excluded."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as orc
+    t0 = time.perf_counter()
+    ot = orc.Tasks(graph_path, 0.9, 0.9, 18, threads=threads)
+    prep = time.perf_counter() - t0
+    try:
+        t = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            res = list(pool.map(lambda x: ot.expand_digest(x[0], x[1], 0, 2), sample_tasks))
+        wall = time.perf_counter() - t
+    finally:
+        ot.close()
+    branches = sum(r[2] for r in res)
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), cpu)
+    except OSError:
+        pass
     return {
-        "workload": "256 directed graphs in one handle, complete enumeration (gamma 0.85, min 10)",
-        "tasks": tasks,
-        "raw_cliques": rows["fifo"][3],
-        "fifo_tasks_per_s": tasks / rows["fifo"][0],
-        "learned_resident_tasks_per_s": tasks / rows["learned_resident"][0],
-        "learned_resident_over_fifo": rows["fifo"][0] / rows["learned_resident"][0],
+        "value": len(sample_tasks) / wall,
+        "unit": "tasks expanded/s",
+        "branches_per_s": branches / wall,
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle on {threads} threads of {cpu}: {len(sample_tasks)} depth-1 tasks of the unit "
+                   f"({branches} branches) in {wall:.1f}s, one task per thread; its preprocessing "
+                   f"({prep:.0f}s) excluded"),
     }
 
 
@@ -140,14 +173,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-tasks", type=int, default=16, help="unit tasks the CPU baseline expands")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-schedules", action="store_true",
-                    help="skip the depth-bounded schedule comparison and the 256-graph batch")
+    ap.add_argument("--no-schedules", action="store_true", help="skip the FIFO / SIZE runs of the unit")
+    ap.add_argument("--no-complete", action="store_true", help="skip the completed workloads (ER 1000, batch)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="run one 256 MiB device-to-device copy before timing (counter calibration under rocprofv3)")
-    ap.add_argument("--only-batch", action="store_true",
-                    help="run only the 256-graph batch comparison and print it (for profiling)")
+                    help="run one 256 MiB device-to-device copy first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): sharding the enumeration over "
@@ -155,14 +186,15 @@ def main():
         sys.exit(2)
 
     import cudqc_amd as dq
+    from cudqc_amd import learn
     data_dir = os.environ.get("DQC_DATA", "/tmp")
-    if args.only_batch:
-        print(json.dumps({"batch": batch_workload(dq, data_dir)}))
-        return
-    graph = os.path.join(data_dir, "dqc_workload2.txt")
+    graph = os.path.join(data_dir, "dqc_workload4.txt")
+    t = time.time()
     if not os.path.exists(graph):
         dq.generate(WORKLOAD, graph)
+    gen_s = time.time() - t
     go, gi, mn = dq.WORKLOADS[WORKLOAD]
+    print(f"bench: R-MAT 2^20 graph ready ({gen_s:.0f}s); preparing", file=sys.stderr, flush=True)
 
     if args.calibrate_copy:
         import torch
@@ -171,42 +203,67 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    eng = dq.Engine()
+    learned = dq.SCHED_LEARNED
+    eng = dq.Engine(unit_config(dq, learned))
     t = time.time()
     k = eng.prepare(graph, go, gi, mn)
     prep_s = time.time() - t
-    finished = False
-    # the plain path: batches in task-stack order, no reordering or splitting
-    plain = dq.SCHED_FIFO
+    root, _, _, ncov = eng.task_meta()[0]
+    eng.set_cost_model(learn.load_model())
+    print(f"bench: {k} condensed vertices ({prep_s:.0f}s); {args.warmup} warm-up steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        if eng.step(1, sched=plain):
-            finished = True
-            break
-    s0 = eng.stats()
-    t0 = time.perf_counter()
-    steps_done = 0
-    for _ in range(args.steps):
-        if finished:
-            break
-        finished = eng.step(1, sched=plain)
-        steps_done += 1
-    t1 = time.perf_counter()
-    s1 = eng.stats()
+        run_unit(eng, root, ncov, learned)
+    ref = None
+    wall = kernel_ms = 0.0
+    tasks = branches = nbytes = launches = 0
+    for i in range(args.steps):
+        w, st, kms = run_unit(eng, root, ncov, learned)
+        counts = (st.tasks_expanded, st.branches, st.raw_cliques, st.truncated_children)
+        if ref is None:
+            ref = counts
+        elif counts != ref:
+            raise RuntimeError(f"unit counts changed between steps: {ref} vs {counts}")
+        wall += w
+        kernel_ms += kms
+        tasks += st.tasks_expanded
+        branches += st.branches
+        nbytes += st.bytes_scanned
+        launches += st.launches
+        print(f"bench: step {i + 1}/{args.steps}: {w:.2f}s", file=sys.stderr, flush=True)
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        n_max, rm_s = maximal_unit(dq, eng, tmp)
+    unit_s = wall / max(args.steps, 1)
+    # a sample of the unit's depth-1 tasks for the CPU baseline (the level-0 window alone)
+    sample = []
+    if not args.no_cpu_baseline:
+        cfg0 = unit_config(dq, dq.SCHED_FIFO)
+        cfg0.max_depth = 0
+        with dq.Engine(cfg0) as e0:
+            e0.prepare(graph, go, gi, mn)
+            e0.step(1)
+            kids = e0.tasks()
+        sample = [(r, nm) for r, nm, _ in sorted(kids, key=lambda x: len(x[0]))][:args.cpu_tasks]
+    sched = {}
+    if not args.no_schedules:
+        for name, sc in (("fifo", dq.SCHED_FIFO), ("size", dq.SCHED_SIZE)):
+            w, st, kms = run_unit(eng, root, ncov, sc)
+            counts = (st.tasks_expanded, st.branches, st.raw_cliques, st.truncated_children)
+            if counts != ref:
+                raise RuntimeError(f"{name} disagrees with the learned form on the unit: {counts} vs {ref}")
+            sched[name] = w
+            print(f"bench: {name} unit: {w:.2f}s", file=sys.stderr, flush=True)
     eng.close()
 
-    wall = t1 - t0
-    tasks = s1.tasks_expanded - s0.tasks_expanded
-    kernel_ms = s1.expand_kernel_ms - s0.expand_kernel_ms
-    nbytes = s1.bytes_scanned - s0.bytes_scanned
     achieved = nbytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
     out = {
-        "metric": "tasks expanded/s (maximal QC enumeration, 1 MI355X, plain order, bit-exact output)",
+        "metric": "tasks expanded/s (maximal QC enumeration, 1 MI355X, batched + learned form, bit-exact output)",
         "value": tasks / wall if wall > 0 else 0.0,
         "unit": "tasks/s",
         "n_gpus": 1,
-        "steps": steps_done,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / max(steps_done, 1),
+        "ms_per_step": unit_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -214,17 +271,21 @@ def main():
         "data": "synthetic (seeded generator)",
         "config": {
             "workload": WORKLOAD_NAME,
-            "step": "one expansion batch (tasks popped from the task-stack top, one kernel launch)",
-            "schedule": "plain FIFO (stack order, no splitting)",
+            "step": "one complete expansion of the unit, from the level-0 task resident in HBM",
+            "schedule": "LEARNED (cost-model ordered batches, long tasks split into pieces)",
             "condensed_vertices": k,
-            "workgroups": s1.workgroups,
-            "tasks_timed": tasks,
-            "branches_timed": s1.branches - s0.branches,
-            "raw_cliques_timed": s1.raw_cliques - s0.raw_cliques,
-            "raw_cliques_per_s": (s1.raw_cliques - s0.raw_cliques) / wall if wall > 0 else 0.0,
-            "kernel_ms_timed": kernel_ms,
+            "workgroups": st.workgroups,
+            "unit_tasks": ref[0],
+            "unit_branches": ref[1],
+            "unit_raw_cliques": ref[2],
+            "unit_maximal_cliques": n_max,
+            "unit_truncated_children": ref[3],
+            "branches_per_s": branches / wall if wall > 0 else 0.0,
+            "maximal_qcs_per_s": n_max / (unit_s + rm_s) if unit_s > 0 else 0.0,
+            "launches_per_step": launches / max(args.steps, 1),
+            "kernel_ms_per_step": kernel_ms / max(args.steps, 1),
             "preprocess_s": prep_s,
-            "enumeration_finished": finished,
+            "generate_s": gen_s,
         },
         "roofline": {
             "bound": "hbm",
@@ -232,18 +293,29 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": PMC_TRAFFIC_PER_ALGO_BYTE * nbytes / max(steps_done, 1),
-            "traffic_src": ("HBM-side bytes per launch: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE ratio to "
-                            "algorithmic bytes (1.670) x this run's algorithmic bytes per launch"),
-            "algorithmic_bytes_per_launch": nbytes / max(steps_done, 1),
-            "kernel_ms_per_launch": kernel_ms / max(steps_done, 1),
+            "traffic": PMC_TRAFFIC_PER_STEP,
+            "kernel": "expand_kernel<true> (HBM-bitmap variant) and its level-0 split launches, all timed steps",
+            "algorithmic_bytes_per_step": nbytes / max(args.steps, 1),
         },
     }
-    if not args.no_schedules:
-        out["schedules"] = schedule_gain(dq, graph)
-        out["batch"] = batch_workload(dq, data_dir)
+    if sched:
+        out["schedules"] = {
+            "workload": "the same unit, complete, identical counts under every schedule",
+            "learned_s": unit_s,
+            "fifo_s": sched["fifo"],
+            "size_s": sched["size"],
+            "fifo_tasks_per_s": ref[0] / sched["fifo"],
+            "size_tasks_per_s": ref[0] / sched["size"],
+            "learned_tasks_per_s": ref[0] / unit_s,
+            "learned_over_fifo": sched["fifo"] / unit_s,
+            "size_over_fifo": sched["fifo"] / sched["size"],
+            "learned_over_size": sched["size"] / unit_s,
+        }
+    if not args.no_complete:
+        out["complete"] = complete_workloads(dq, data_dir)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(graph, args.cpu_seconds)
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(graph, sample, threads)
     print(json.dumps(out))
 
 

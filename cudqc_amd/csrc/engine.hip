@@ -1851,6 +1851,10 @@ struct dqc_handle {
     float* d_cost = nullptr;       // per-task predictions of the current batch
     std::vector<float> hpred;
     std::vector<void*> allocs;
+    // batch buffer and task stack capacities left to the library (dqc_config field 0):
+    // sized from the graph when it is uploaded
+    bool auto_pool_v = false, auto_spill_t = false, auto_spill_v = false;
+    unsigned long long cap_pool_v = 0, cap_spill_t = 0, cap_spill_v = 0;  // allocated
 };
 
 namespace {
@@ -1924,6 +1928,9 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (cfg) h->cfg = *cfg;
     else h->cfg.host_addfail_check = 1;
     dqc_config& c = h->cfg;
+    h->auto_pool_v = c.pool_vertices == 0;
+    h->auto_spill_t = c.spill_tasks == 0;
+    h->auto_spill_v = c.spill_vertices == 0;
     if (!c.pool_vertices) c.pool_vertices = 1ull << 27;
     if (!c.spill_tasks) c.spill_tasks = 1ull << 25;
     if (!c.spill_vertices) c.spill_vertices = 1ull << 30;
@@ -1958,9 +1965,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!hip_ok(h, hipEventCreate(&h->ev0), "hipEventCreate")) return DQC_E_HIP;
     if (!hip_ok(h, hipEventCreate(&h->ev1), "hipEventCreate")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_bdesc, c.pool_tasks, "batch alloc")) return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_bvtx, c.pool_vertices, "batch alloc")) return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_sdesc, c.spill_tasks, "stack alloc")) return DQC_E_HIP;
-    if (!dev_alloc(h, &h->d_svtx, c.spill_vertices, "stack alloc")) return DQC_E_HIP;
+    // the batch records and the task stack are allocated with the graph (size_buffers)
     if (!dev_alloc(h, &h->d_clq_ids, c.clique_ids, "clique alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_clq_meta, c.clique_sets, "clique alloc")) return DQC_E_HIP;
     if (!dev_alloc(h, &h->d_q, 1, "counter alloc") || !dev_alloc(h, &h->d_g, 1, "counter alloc"))
@@ -1988,6 +1993,41 @@ dqc_status dqc_close(dqc_handle* h) {
 const char* dqc_last_error(const dqc_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
 }  // extern "C"
+
+// The batch-record buffer and the task stack, sized from the largest graph (kloc) unless
+// dqc_config set them: a stack of 8192 x kloc records (within [2^22, 2^30]; a task holds
+// at most kloc records and the depth-first pop keeps the stack near the enumeration
+// depth times the batch), tasks 1/16 of that, a batch buffer of 1024 x kloc records
+// (within [2^20, 2^27]). Small graphs then take a few hundred MB instead of tens of GB.
+static dqc_status size_buffers(dqc_handle* h) {
+    dqc_config& c = h->cfg;
+    const unsigned long long k = (unsigned long long)std::max(h->kloc, 1);
+    auto clampu = [](unsigned long long x, unsigned long long lo, unsigned long long hi) {
+        return std::max(lo, std::min(hi, x));
+    };
+    if (h->auto_spill_v) c.spill_vertices = clampu(8192ull * k, 1ull << 22, 1ull << 30);
+    if (h->auto_spill_t) c.spill_tasks = clampu(c.spill_vertices / 16, 1ull << 16, 1ull << 25);
+    if (h->auto_pool_v) c.pool_vertices = clampu(1024ull * k, 1ull << 20, 1ull << 27);
+    auto realloc_buf = [&](auto** p, unsigned long long& cap, unsigned long long want, const char* what) {
+        if (*p && cap == want) return true;
+        if (*p) {
+            hipFree(*p);
+            h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)*p), h->allocs.end());
+            *p = nullptr;
+        }
+        cap = 0;
+        if (!dev_alloc(h, p, want, what)) return false;
+        cap = want;
+        return true;
+    };
+    const bool moved = h->cap_spill_t != c.spill_tasks || h->cap_spill_v != c.spill_vertices;
+    if (!realloc_buf(&h->d_bvtx, h->cap_pool_v, c.pool_vertices, "batch alloc") ||
+        !realloc_buf(&h->d_sdesc, h->cap_spill_t, c.spill_tasks, "stack alloc") ||
+        !realloc_buf(&h->d_svtx, h->cap_spill_v, c.spill_vertices, "stack alloc"))
+        return DQC_E_HIP;
+    if (moved) h->st_t = h->st_v = h->def_t = h->def_v = 0;  // a new stack holds no task
+    return DQC_OK;
+}
 
 // Uploads n ids' CSR lists; kloc is the largest vertex count of one graph (n itself for
 // a single graph; batched graphs share the id space with 32-aligned bases).
@@ -2045,7 +2085,7 @@ static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint
     if (h->big && !hip_ok(h, hipMemset(h->d_scratch, 0, sizeof(uint32_t) * h->stride * (size_t)(h->launch_wg + 1)), "scratch clear"))
         return DQC_E_HIP;
     h->st.n_condensed = n;
-    return DQC_OK;
+    return size_buffers(h);
 }
 
 extern "C" {

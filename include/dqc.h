@@ -27,7 +27,9 @@ typedef int32_t dqc_status;
 typedef struct dqc_handle dqc_handle;
 
 /* Capacities are advisory in the way DS_Sizes.csv is for the reference
- * (reference src/common.cpp:521-557); 0 selects a default. */
+ * (reference src/common.cpp:521-557); 0 selects a default. The batch buffer and task
+ * stack defaults follow the largest graph uploaded (k vertices): 1024 k batch records
+ * (2^20 .. 2^27), 8192 k stack records (2^22 .. 2^30), stack tasks 1/16 of that. */
 typedef struct {
     int32_t device;            /* HIP device ordinal                                  */
     int32_t workgroups;        /* persistent workgroups; 0 = as many as are resident  */
