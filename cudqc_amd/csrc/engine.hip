@@ -284,7 +284,8 @@ struct W {
     uint32_t nbranch;             // branches run for the current task / piece
     uint32_t launch_steps;        // stack checks this workgroup passed in this launch
     unsigned long long t_start;   // thread 0: realtime counter at the launch's start (time slices)
-    bool discard_child;           // the current level-0 branch lies before root_first
+    bool discard_child;           // the current level-0 branch lies before root_first: it only
+                                  // tells whether it writes a child (no child, clique or count)
     Field pom, poc, pim, pic, com, coc, cim, cic;  // parent and child degrees (4 per id)
     uint32_t *pl2, *adjc;
     int32_t *pids, *mem, *la, *lb, *lc, *lst, *crit, *gat;
@@ -614,25 +615,52 @@ struct W {
         return c;
     }
 
+    // Number of entries of nbr[b, e) that are current candidates, for a group of G lanes
+    // (lanes sub = 0..G-1 of one vertex's group): each lane takes every G-th entry, four
+    // at a time, and the group's lanes read G consecutive entries per load, so a load
+    // instruction touches few cache lines; the sum is formed in the group (G a power of 2,
+    // groups aligned inside the wavefront).
+    template <int G>
+    __device__ __forceinline__ uint32_t count_cand_group(const int32_t* nbr, unsigned long long b, unsigned long long e,
+                                                         int sub) const {
+        uint32_t c = 0;
+        unsigned long long i = b + (unsigned long long)sub;
+        for (; i + 3 * G < e; i += 4 * G) {
+            const int a0 = nbr[i], a1 = nbr[i + G], a2 = nbr[i + 2 * G], a3 = nbr[i + 3 * G];
+            const uint32_t w0 = bword(ccand, a0), w1 = bword(ccand, a1), w2 = bword(ccand, a2), w3 = bword(ccand, a3);
+            c += bbit(w0, a0) + bbit(w1, a1) + bbit(w2, a2) + bbit(w3, a3);
+        }
+        for (; i < e; i += G) c += (uint32_t)bt(ccand, nbr[i]);
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        return c;
+    }
+
     // can-degrees of live vertices recounted as |out(x) n R|, |in(x) n R| (R = ccand).
-    // Vertices with short lists are counted one per thread; vertices with long lists
-    // (hubs) are queued and counted one per wavefront.
+    // Each vertex is counted by a group of RG lanes (coalesced list reads, many vertices
+    // in flight); vertices with long lists (hubs) are queued and counted one per wavefront.
     __device__ __forceinline__ void recount(int cm, const int32_t* R, int nr) {
-        constexpr unsigned long long LONG_LIST = 192;
+        constexpr unsigned long long LONG_LIST = 512;
+        constexpr int RG = 8;
         const unsigned long long *oo = P->oo, *io = P->io;
         const int N = cm + nr;
         if (tid == 0) S->cnt[8] = 0;
         __syncthreads();
-        for (int i = tid; i < N; i += NT) {
+        const int sub = tid & (RG - 1);
+        for (int i = tid / RG; i < N; i += NT / RG) {
             const int x = i < cm ? mem[i] : R[i - cm];
             const unsigned long long ob = oo[x], oe = oo[x + 1], ib = io[x], ie = io[x + 1];
-            bytes += 4ull * ((oe - ob) + (ie - ib)) + 32;
             if ((oe - ob) + (ie - ib) > LONG_LIST) {
-                gat[atomicAdd(&S->cnt[8], 1)] = x;
+                if (sub == 0) gat[atomicAdd(&S->cnt[8], 1)] = x;
                 continue;
             }
-            coc[x] = count_cand(P->on, ob, oe);
-            cic[x] = count_cand(P->in, ib, ie);
+            const uint32_t co = count_cand_group<RG>(P->on, ob, oe, sub);
+            const uint32_t ci = count_cand_group<RG>(P->in, ib, ie, sub);
+            if (sub == 0) {
+                coc[x] = co;
+                cic[x] = ci;
+                bytes += 4ull * ((oe - ob) + (ie - ib)) + 32;
+            }
         }
         __syncthreads();
         const int nlong = S->cnt[8];
@@ -641,7 +669,11 @@ struct W {
             const int x = gat[j];
             const uint32_t co = count_cand_wave(P->on, oo[x], oo[x + 1], lane);
             const uint32_t ci = count_cand_wave(P->in, io[x], io[x + 1], lane);
-            if (lane == 0) { coc[x] = co; cic[x] = ci; }
+            if (lane == 0) {
+                coc[x] = co;
+                cic[x] = ci;
+                bytes += 4ull * ((oo[x + 1] - oo[x]) + (io[x + 1] - io[x])) + 32;
+            }
         }
         __syncthreads();
     }
@@ -714,7 +746,7 @@ struct W {
 
     // reference src/device_kernels.cu:1488-1532 (d_check_for_clique)
     __device__ __forceinline__ void check_clique(int cm) {
-        if (cm < P->min_size) return;
+        if (cm < P->min_size || discard_child) return;  // a branch before a level-0 window emits nothing
         uint32_t mo = (uint32_t)P->mdo[cm], mi = (uint32_t)P->mdi[cm];
         int bad = 0;
         for (int a = tid; a < cm; a += NT) {
@@ -1144,7 +1176,7 @@ struct W {
         int nr = read_counter(4);
         int ub = 0, lbd = 0, meo = 0, mei = 0;
         int32_t* R = la;
-        if (tid == 0) atomicAdd(&P->g->branches, 1ull);
+        if (tid == 0 && !discard_child) atomicAdd(&P->g->branches, 1ull);  // window branches only
         tick(3);
         bool ok = degree_prune(cm, &R, &nr, tot - 1 - nm, ub, lbd, meo, mei, lb, lc);
         tick(4);

@@ -578,9 +578,13 @@ struct Expander {
 
     // One task's branch loop: reference src/host_functions.cu:987-1134 (host levels)
     // and src/device_kernels.cu:25-173 (device levels).
-    // max_steps > 0: only the loop steps j < ncov + max_steps run (a bounded prefix of
-    // the same loop, the library's root_branches unit of work).
-    void expand(const Rec* task, int tot, Mode mode, int max_steps = 0) {
+    // max_steps > 0: only the loop steps j < ncov + first_step + max_steps run (a bounded
+    // prefix of the same loop, the library's root_branches unit of work). first_step > 0:
+    // the steps before ncov + first_step (a window's head) still remove and test
+    // lookahead, but their branches run only until one of them produces a child, which
+    // is then the first child (host level-1 role) and is dropped, as are the cliques,
+    // children and branch counts of every head branch.
+    void expand(const Rec* task, int tot, Mode mode, int max_steps = 0, int first_step = 0) {
         ensure(tot);
         sink->st.tasks++;
         sink->st.task_hist[31 - __builtin_clz((unsigned)tot | 1)]++;
@@ -593,8 +597,11 @@ struct Expander {
         if (mode != DEVICE)
             while (nm + ncov < tot && rv[nm + ncov].label == 2) ++ncov;
         int nc = tot - nm;
-        const int expansions = max_steps > 0 ? std::min(nc, ncov + max_steps) : nc;
+        const int expansions = max_steps > 0 ? std::min(nc, ncov + first_step + max_steps) : nc;
         std::vector<Rec> child(tot);
+        bool head_child = false;  // a head branch produced a child
+        Sink* const real_sink = sink;
+        Sink scratch;
         for (int j = ncov; j < expansions; ++j) {
             if (j != ncov) {
                 int r = remove_one(rv, tot, nc, nm);
@@ -605,29 +612,46 @@ struct Expander {
                 emit_clique(rv, tot);
                 break;
             }
-            Rec* cv = child.data();
-            int cm = nm, cc = nc, ct = tot;
-            for (int i = 0; i < nm; ++i) cv[i] = rv[i];
-            cv[nm] = rv[tot - 1];
-            for (int i = nm; i < tot - 1; ++i) cv[i + 1] = rv[i];
-            for (int i = nm + 1; i <= nm + ncov; ++i) cv[i].label = 0;
-            int ub = 0, lb = 0, meo = 0, mei = 0;
-            sink->st.branches++;
-            if (!add_one(cv, ct, cc, cm, ub, lb, meo, mei)) {
-                if (mode == DEVICE || P.host_addfail_check) check_clique(cv, cm);
-                continue;
+            const bool head = j - ncov < first_step;
+            if (head && head_child) continue;
+            // a head branch writes into a scratch sink: only whether it yields a child counts
+            if (head) {
+                scratch = Sink();
+                sink = &scratch;
             }
-            int s = critical(cv, ct, cc, cm, ub, lb, meo, mei);
-            if (s == 2) continue;
-            check_clique(cv, cm);
-            if (s == 0) continue;
-            std::sort(cv, cv + ct, q_less);
-            if (cc > 0) {
-                sink->kids.emplace_back(cv, cv + ct);
-                sink->st.children++;
-                sink->st.kid_hist[31 - __builtin_clz((unsigned)ct | 1)]++;
-                sink->st.bytes += 24ull * ct;
+            run_branch(rv, child.data(), tot, nm, nc, ncov, mode);
+            if (head) {
+                head_child = !scratch.kids.empty();
+                sink = real_sink;
             }
+        }
+    }
+
+    // The branch of the current step: child = members + the last vertex + the rest, then
+    // add, pruning, critical, clique check and the child (reference
+    // src/host_functions.cu:1046-1133, src/device_kernels.cu:92-165).
+    void run_branch(Rec* rv, Rec* cv, int tot, int nm, int nc, int ncov, Mode mode) {
+        int cm = nm, cc = nc, ct = tot;
+        for (int i = 0; i < nm; ++i) cv[i] = rv[i];
+        cv[nm] = rv[tot - 1];
+        for (int i = nm; i < tot - 1; ++i) cv[i + 1] = rv[i];
+        for (int i = nm + 1; i <= nm + ncov; ++i) cv[i].label = 0;
+        int ub = 0, lb = 0, meo = 0, mei = 0;
+        sink->st.branches++;
+        if (!add_one(cv, ct, cc, cm, ub, lb, meo, mei)) {
+            if (mode == DEVICE || P.host_addfail_check) check_clique(cv, cm);
+            return;
+        }
+        int s = critical(cv, ct, cc, cm, ub, lb, meo, mei);
+        if (s == 2) return;
+        check_clique(cv, cm);
+        if (s == 0) return;
+        std::sort(cv, cv + ct, q_less);
+        if (cc > 0) {
+            sink->kids.emplace_back(cv, cv + ct);
+            sink->st.children++;
+            sink->st.kid_hist[31 - __builtin_clz((unsigned)ct | 1)]++;
+            sink->st.bytes += 24ull * ct;
         }
     }
 };
@@ -1046,7 +1070,8 @@ int orc_root(void* hv, uint32_t* out, int* ncov) {
 // Expands one task one level. Children: *kid_meta gets 2 int64 per child {record offset,
 // size}, members counted by label; *kids the records; cliques as "len, ids..." int32.
 // All outputs malloc'd (orc_free). Returns the number of children.
-int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int mode, int max_steps, uint32_t** kids,
+int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int mode, int max_steps, int first_step,
+               uint32_t** kids,
                int64_t** kid_meta, int32_t** kid_nm, int32_t** clq, int64_t* clq_len, int64_t* branches) {
     using namespace orc;
     auto* h = (orc_handle*)hv;
@@ -1059,7 +1084,7 @@ int orc_expand(void* hv, const uint32_t* rec, int nvert, int nm, int ncov, int m
     Sink sink;
     Expander ex(h->g, h->P);
     ex.sink = &sink;
-    ex.expand(t.data(), nvert, (Mode)mode, max_steps);
+    ex.expand(t.data(), nvert, (Mode)mode, max_steps, first_step);
     size_t tot = 0;
     for (auto& k : sink.kids) tot += k.size();
     *kids = (uint32_t*)std::malloc(sizeof(uint32_t) * 6 * (tot + 1));

@@ -59,6 +59,7 @@ def load():
     PU32 = ctypes.POINTER(ctypes.c_uint32)
     lib.orc_expand.restype = ctypes.c_int
     lib.orc_expand.argtypes = [ctypes.c_void_p, PU32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int,
                                ctypes.POINTER(PU32), ctypes.POINTER(P64), ctypes.POINTER(P32), ctypes.POINTER(P32),
                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     _lib = lib
@@ -119,9 +120,11 @@ class Tasks:
         self.lib.orc_root(self.h, arr, ctypes.byref(ncov))
         return [tuple(arr[6 * i:6 * i + 6]) for i in range(k)], ncov.value
 
-    def expand(self, records, nm, ncov=0, mode=2, max_steps=0):
+    def expand(self, records, nm, ncov=0, mode=2, max_steps=0, first_step=0):
         """Returns (children as (records, nm), cliques as sorted id tuples, branches);
-        max_steps > 0 runs only the first max_steps loop steps after the covered prefix."""
+        max_steps > 0 runs only the loop steps [first_step, first_step + max_steps) after the
+        covered prefix (the steps before first_step remove and test lookahead; their branches
+        run only until one yields a child, and give no output)."""
         flat = (ctypes.c_uint32 * max(6 * len(records), 1))(*[x for r in records for x in r])
         kids = ctypes.POINTER(ctypes.c_uint32)()
         meta = ctypes.POINTER(ctypes.c_int64)()
@@ -129,7 +132,8 @@ class Tasks:
         clq = ctypes.POINTER(ctypes.c_int)()
         clen = ctypes.c_int64()
         br = ctypes.c_int64()
-        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), ctypes.byref(kids), ctypes.byref(meta),
+        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), int(first_step),
+                                ctypes.byref(kids), ctypes.byref(meta),
                                 ctypes.byref(knm), ctypes.byref(clq), ctypes.byref(clen), ctypes.byref(br))
         out = []
         for c in range(n):
@@ -144,7 +148,7 @@ class Tasks:
             self.lib.orc_free(x)
         return out, sorted(cl), br.value
 
-    def expand_digest(self, records, nm, ncov=0, mode=2, max_steps=0):
+    def expand_digest(self, records, nm, ncov=0, mode=2, max_steps=0, first_step=0):
         """As expand, with each child as (SHA-1 of its records' bytes, nm): records are 6
         uint32 per vertex, the layout the library's task stack uses."""
         import hashlib
@@ -155,9 +159,9 @@ class Tasks:
         clq = ctypes.POINTER(ctypes.c_int)()
         clen = ctypes.c_int64()
         br = ctypes.c_int64()
-        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), ctypes.byref(kids),
-                                ctypes.byref(meta), ctypes.byref(knm), ctypes.byref(clq), ctypes.byref(clen),
-                                ctypes.byref(br))
+        n = self.lib.orc_expand(self.h, flat, len(records), nm, ncov, mode, int(max_steps), int(first_step),
+                                ctypes.byref(kids), ctypes.byref(meta), ctypes.byref(knm), ctypes.byref(clq),
+                                ctypes.byref(clen), ctypes.byref(br))
         total = sum(meta[2 * c + 1] for c in range(n))
         buf = ctypes.string_at(kids, 24 * total) if total else b""
         out = []

@@ -20,7 +20,9 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 GO, GI, MN = 0.9, 0.9, 18
-STEPS = 40  # level-0 loop steps of the window
+STEPS = 40  # the first level-0 loop steps (they hold the first child, the host level-1 task)
+# a window of level-0 steps where most branches yield a depth-1 task of 10-20k vertices
+WINDOW = (90112, 12)
 
 
 @pytest.fixture(scope="module")
@@ -39,14 +41,16 @@ def oracle_w4(w4):
     root, ncov = t.root()
     print(f"oracle: {len(root)} condensed vertices, {ncov} covered; expanding {STEPS} level-0 steps", flush=True)
     kids, cliques, branches = t.expand(root, 0, ncov, orc.Tasks.HOST_L0, max_steps=STEPS)
-    yield t, root, ncov, kids, cliques, branches
+    print(f"oracle: level-0 window {WINDOW}", flush=True)
+    wkids, wcl, wbr = t.expand(root, 0, ncov, orc.Tasks.HOST_L0, max_steps=WINDOW[1], first_step=WINDOW[0])
+    yield t, root, ncov, kids, cliques, branches, (wkids, wcl, wbr)
     t.close()
 
 
 def test_level0_window_matches_oracle(dqc, w4, oracle_w4):
     """The first level-0 steps, branch-parallel on the GPU, equal the oracle's sequential
     loop over the same steps (the bounded unit of work root_branches defines)."""
-    _, root, ncov, okids, ocl, obr = oracle_w4
+    _, root, ncov, okids, ocl, obr, _ = oracle_w4
     cfg = dqc.Config()
     cfg.host_addfail_check = 1
     cfg.root_branches = STEPS
@@ -66,18 +70,37 @@ def test_level0_window_matches_oracle(dqc, w4, oracle_w4):
     assert sorted(tuple(sorted(c)) for c in cl) == ocl
 
 
+def test_level0_window_in_dense_part_matches_oracle(dqc, w4, oracle_w4):
+    """Level-0 steps [90112, 90124) (the bounded unit of work root_first / root_branches
+    define; the steps before run only until the first child): the same children, cliques
+    and branch count as the oracle's loop over that window, all children device tasks."""
+    _, root, ncov, _, _, _, (wkids, wcl, wbr) = oracle_w4
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.root_first, cfg.root_branches = WINDOW
+    with dqc.Engine(cfg) as eng:
+        eng.prepare(w4, GO, GI, MN)
+        eng.step(1)
+        tasks = eng.tasks()
+        cl = eng.cliques()
+        st = eng.stats()
+    print(f"window: {len(wkids)} children, {wbr} branches, {len(wcl)} cliques", flush=True)
+    assert len(wkids) >= 8 and st.branches == wbr
+    assert sorted((tuple(r), nm) for r, nm, _ in tasks) == sorted((tuple(r), nm) for r, nm in wkids)
+    assert all(m == 2 for _, _, m in tasks)
+    assert sorted(tuple(sorted(c)) for c in cl) == wcl
+
+
 def test_depth1_tasks_match_oracle(dqc, w4, oracle_w4):
-    """A deterministic sample of the window's depth-1 tasks (the first child, which holds
-    nearly the whole graph, is left to the level-0 test: its 124k branches are beyond the
-    oracle's time here), each split into pieces on the GPU and compared with the oracle's
-    unsplit expansion."""
-    ot, _, _, okids, _, _ = oracle_w4
-    rest = sorted(range(1, len(okids)), key=lambda i: len(okids[i][0]))
-    sample = sorted(set(rest[:6] + rest[len(rest) // 3:len(rest) // 3 + 2] + rest[-2:]))
-    assert len(sample) >= 8, len(okids)
-    print("sample sizes", [len(okids[i][0]) for i in sample], flush=True)
-    with ThreadPoolExecutor(max_workers=len(sample)) as pool:  # the oracle runs meanwhile
-        futs = {i: pool.submit(ot.expand_digest, okids[i][0], okids[i][1], 0, orc.Tasks.DEVICE) for i in sample}
+    """The depth-1 tasks of the dense window, each split into pieces on the GPU (so the
+    whole chip runs it) and compared with the oracle's unsplit expansion (the oracle runs
+    them side by side on the host cores meanwhile)."""
+    ot, _, _, _, _, _, (wkids, _, _) = oracle_w4
+    sample = list(range(len(wkids)))[:10]
+    assert len(sample) >= 8, len(wkids)
+    print("sample sizes", [len(wkids[i][0]) for i in sample], flush=True)
+    with ThreadPoolExecutor(max_workers=len(sample)) as pool:
+        futs = {i: pool.submit(ot.expand_digest, wkids[i][0], wkids[i][1], 0, orc.Tasks.DEVICE) for i in sample}
         cfg = dqc.Config()
         cfg.host_addfail_check = 1
         cfg.schedule = dqc.SCHED_SIZE
@@ -86,13 +109,14 @@ def test_depth1_tasks_match_oracle(dqc, w4, oracle_w4):
         with dqc.Engine(cfg) as eng:
             eng.prepare(w4, GO, GI, MN)
             for i in sample:
-                recs, nm = okids[i]
+                recs, nm = wkids[i]
                 eng.reset()
                 eng.push_task(recs, nm, 0, orc.Tasks.DEVICE)
                 eng.step(1)  # the split pass: pieces only
                 assert eng.stats().tasks_expanded == 1 and eng.stats().branches == 0, i
-                eng.step(1)  # every piece
+                eng.step(1)  # every piece (one batch holds them all)
                 st = eng.stats()
+                assert st.stack_pushbacks == 0 and st.time_slices == 0, i
                 got[i] = (sorted((h, m) for h, m, _ in eng.task_digests()),
                           sorted(tuple(sorted(c)) for c in eng.cliques()), st.branches, st.tasks_expanded)
                 print(f"task {i}: size {len(recs)}, {st.branches} branches, {len(got[i][0])} children", flush=True)

@@ -257,3 +257,27 @@ def test_expand_digest_matches_expand(oracle_lib, manifest):
         assert len(sub) <= len(full)
     finally:
         t.close()
+
+
+def test_expand_window_is_prefix_difference(oracle_lib, manifest):
+    """A window [a, a + b) of a task's loop steps (first_step / max_steps) yields exactly
+    the children, cliques and branches the prefix [0, a + b) yields beyond the prefix
+    [0, a): the head's branches only decide the first child and leave no output."""
+    from collections import Counter
+    from tests.conftest import GOLDEN
+    for name, a, b in (("workload3_g2", 3, 30), ("workload3_g0", 5, 20), ("rand069", 2, 9)):
+        c = next(x for x in manifest["cases"] if x["name"] == name)
+        t = oracle_lib.Tasks(os.path.join(GOLDEN, c["graph"]), c["out_gamma"], c["in_gamma"], c["min_size"], threads=1)
+        try:
+            root, ncov = t.root()
+            HL0 = oracle_lib.Tasks.HOST_L0
+            full = t.expand(root, 0, ncov, HL0, max_steps=a + b)
+            head = t.expand(root, 0, ncov, HL0, max_steps=a)
+            win = t.expand(root, 0, ncov, HL0, max_steps=b, first_step=a)
+            key = lambda kids: Counter((tuple(r), nm) for r, nm in kids)
+            assert key(win[0]) == key(full[0]) - key(head[0]), name
+            assert Counter(win[1]) == Counter(full[1]) - Counter(head[1]), name
+            assert win[2] == full[2] - head[2], name
+            assert head[0], name  # the head holds a child here: the window's children keep device roles
+        finally:
+            t.close()
