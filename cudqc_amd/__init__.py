@@ -23,7 +23,7 @@ EXPORTS = [
     "dqc_task_features", "dqc_mlp_forward", "dqc_mlp_gradient", "dqc_mlp_train", "dqc_set_cost_model",
     "dqc_upload_graph_batch", "dqc_push_graph_root", "dqc_fetch_batch_cliques", "dqc_prepare_batch",
     "dqc_search_batch", "dqc_min_degrees", "dqc_search_graph", "dqc_prepare_graph", "dqc_save_graph",
-    "dqc_expand_noalloc", "dqc_batch_size", "dqc_finish_batch",
+    "dqc_expand_noalloc", "dqc_batch_size", "dqc_finish_batch", "dqc_set_unit",
 ]
 
 MLP_IN, MLP_HIDDEN = 7, 64
@@ -174,6 +174,8 @@ def load():
                                                                                 ctypes.c_int32, P32, P32]
     lib.dqc_finish_batch.restype = ctypes.c_int32
     lib.dqc_finish_batch.argtypes = [ctypes.c_void_p, PPC, PI64, ctypes.POINTER(Stats)]
+    lib.dqc_set_unit.restype = ctypes.c_int32
+    lib.dqc_set_unit.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.dqc_batch_size.restype = ctypes.c_int32
     lib.dqc_batch_size.argtypes = [ctypes.c_void_p, P32]
     lib.dqc_save_graph.restype = ctypes.c_int32
@@ -403,6 +405,11 @@ class Engine:
             out.append((g[i], list(ids[p:p + sizes[i]])))
             p += sizes[i]
         return out
+
+    def set_unit(self, root_first, root_branches, max_depth):
+        """dqc_set_unit: the bounded unit of work for tasks queued from now on."""
+        self._check(self.lib.dqc_set_unit(self.h, int(root_first), int(root_branches), int(max_depth)),
+                    "dqc_set_unit")
 
     def finish_batch(self, max_paths):
         """dqc_finish_batch: each graph's maximal file (None entries skip); returns

@@ -2860,6 +2860,15 @@ dqc_status dqc_fetch_batch_cliques(dqc_handle* h, int64_t* count, int64_t* total
     return DQC_OK;
 }
 
+dqc_status dqc_set_unit(dqc_handle* h, int32_t root_first, int32_t root_branches, int32_t max_depth) {
+    if (!h || root_first < 0 || root_branches < 0 || max_depth < 0) return DQC_E_ARG;
+    drop_graphs(h);  // captured steps hold the old kernel parameters
+    h->cfg.root_first = root_first;
+    h->cfg.root_branches = root_branches;
+    h->cfg.max_depth = max_depth;
+    return DQC_OK;
+}
+
 dqc_status dqc_batch_size(const dqc_handle* h, int32_t* ngraphs) {
     if (!h || !ngraphs) return DQC_E_ARG;
     *ngraphs = h->ngraphs;

@@ -305,6 +305,10 @@ void dqc_free(void* p);
  * graph g's condensed size. */
 dqc_status dqc_prepare_batch(dqc_handle* h, const char* const* graph_paths, int32_t ngraphs, double out_gamma,
                              double in_gamma, int32_t min_size, int32_t* n_condensed);
+/* Re-aims an open handle's bounded unit of work (the dqc_config fields root_first,
+ * root_branches and max_depth) for the tasks queued from now on, e.g. to run several
+ * windows of one prepared graph (dqc_reset, then dqc_push_root again). */
+dqc_status dqc_set_unit(dqc_handle* h, int32_t root_first, int32_t root_branches, int32_t max_depth);
 /* Graphs in the handle's id space (1 for a single graph). */
 dqc_status dqc_batch_size(const dqc_handle* h, int32_t* ngraphs);
 /* Result dump of a batched run (the per-graph h_dump_cliques + RemoveNonMax of the
