@@ -49,6 +49,7 @@ class Config(ctypes.Structure):
         ("schedule", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("resident_steps", ctypes.c_int32),
         ("root_branches", ctypes.c_int32), ("split_pieces", ctypes.c_int32), ("root_first", ctypes.c_int32),
         ("launch_ms", ctypes.c_int32),
+        ("wave_tasks", ctypes.c_int32),
     ]
 
 
@@ -64,6 +65,7 @@ class Stats(ctypes.Structure):
         ("truncated_children", ctypes.c_int64), ("stack_pushbacks", ctypes.c_int64),
         ("peak_stack_tasks", ctypes.c_int64), ("peak_stack_records", ctypes.c_int64),
         ("preprocess_ms", ctypes.c_double), ("rmnonmax_ms", ctypes.c_double), ("time_slices", ctypes.c_int64),
+        ("wave_launches", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -51,6 +51,11 @@ constexpr int KMAX = KW * 32;
 #endif
 constexpr int SORT_N = DQC_SORT_N;  // LDS sort capacity (larger children sort in HBM)
 constexpr int HMAX = 256;          // member-degree histogram bins (members < HMAX)
+// Wavefront-per-task kernel: one 64-lane wavefront runs one task, with LDS id bitmaps of
+// WAVE_KW words (condensed graphs of at most WAVE_KMAX vertices).
+constexpr int WAVE_T = 64;
+constexpr int WAVE_KW = 64;
+constexpr int WAVE_KMAX = WAVE_KW * 32;
 #ifndef DQC_SCAN_UNR
 #define DQC_SCAN_UNR 8
 #endif
@@ -233,7 +238,9 @@ struct Field {
 };
 
 // All functions below are executed by every thread of the workgroup with uniform
-// control flow; `W` bundles the per-workgroup pointers and uniform scalars.
+// control flow; `W` bundles the per-workgroup pointers and uniform scalars. NT threads
+// run one task: a 256-thread workgroup, or one wavefront (64) for small tasks.
+template <int NT>
 struct W {
     const KParams* P;
     Shared* S;
@@ -1410,12 +1417,16 @@ struct W {
 #ifndef DQC_WAVES_PER_EU
 #define DQC_WAVES_PER_EU 8
 #endif
-template <bool BIG>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DQC_WAVES_PER_EU)))
+// TPB threads per task (a 256-thread workgroup, or one wavefront) and LKW words per LDS
+// id bitmap (the LDS-bitmap variant covers condensed ids below 32 * LKW).
+// The wavefront kernel's LDS (8 KB per wavefront) allows 5 waves per SIMD, and its
+// register budget is set for that many.
+template <bool BIG, int TPB = NT, int LKW = KW>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(TPB == WAVE_T ? 5 : DQC_WAVES_PER_EU)))
 void expand_kernel(KParams P) {
     if (P.in_ntask == ~0ull && blockIdx.x >= P.q->wg_limit) return;  // no stack room for them
     __shared__ Shared S;
-    W w;
+    W<TPB> w;
     w.P = &P;
     w.S = &S;
     w.tid = threadIdx.x;
@@ -1435,19 +1446,19 @@ void expand_kernel(KParams P) {
     if constexpr (BIG) {
         bm = base + 22 * k;  // kept all-zero between tasks; zeroed when the graph is uploaded
         __shared__ uint32_t lds_sum[SUMW];
-        for (int i = threadIdx.x; i < SUMW; i += NT) lds_sum[i] = 0;
+        for (int i = threadIdx.x; i < SUMW; i += TPB) lds_sum[i] = 0;
         w.lsum = lds_sum;
         int gs = P.sum_shift;  // ids per summary bit: 2^gs, so that every graph-relative id has a bit
         while (((unsigned long long)(P.kloc > 0 ? P.kloc - 1 : 0) >> gs) >= 32ull * SUMW) ++gs;
         w.gsh = gs;
     } else {
-        __shared__ uint32_t lds_bm[NBITMAPS * KW];
+        __shared__ uint32_t lds_bm[NBITMAPS * LKW];
         bm = lds_bm;
-        for (int i = threadIdx.x; i < NBITMAPS * KW; i += NT) lds_bm[i] = 0;
+        for (int i = threadIdx.x; i < NBITMAPS * LKW; i += TPB) lds_bm[i] = 0;
         w.lsum = nullptr;
         w.gsh = 0;
     }
-    const int bw = BIG ? w.kw : KW;
+    const int bw = BIG ? w.kw : LKW;
     w.plive = bm; w.pmem = bm + bw; w.cmem = bm + 2 * bw; w.ccand = bm + 3 * bw; w.cL = bm + 4 * bw; w.ccrit = bm + 5 * bw;
     w.dirty_m = 0;
     w.dirty_l = 0;
@@ -1833,6 +1844,9 @@ struct dqc_handle {
     std::vector<int32_t> gbase;
     bool big = false;    // condensed graph above KMAX vertices: HBM id bitmaps
     int launch_wg = 0;   // workgroups per launch (the scratch bounds it for large graphs)
+    int wave_wg = 0;     // resident single-wavefront workgroups of the wavefront-per-task kernel
+    bool batch_wave = false;  // the popped batch runs one task per wavefront
+    int64_t wave_launches = 0;
     unsigned long long *d_oo = nullptr, *d_io = nullptr, *d_to = nullptr;
     int32_t *d_on = nullptr, *d_in = nullptr, *d_tn = nullptr;
     int32_t *d_mdo = nullptr, *d_mdi = nullptr, *d_ubo = nullptr, *d_ubi = nullptr;
@@ -1969,6 +1983,7 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
     if (!c.clique_sets) c.clique_sets = 1ull << 25;
     if (!c.clique_ids) c.clique_ids = 1ull << 29;
     if (!c.split_branches) c.split_branches = 1024;
+    if (!c.wave_tasks) c.wave_tasks = DQC_WAVE_TASKS_DEFAULT;
     if (c.split_chunk <= 0) c.split_chunk = 32;
     if (c.schedule == DQC_SCHED_DEFAULT) c.schedule = DQC_SCHED_SIZE;
     h->device = c.device;
@@ -1991,6 +2006,11 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
         return DQC_E_HIP;
     if (per_cu < 1) per_cu = 1;
     h->workgroups = c.workgroups > 0 ? c.workgroups : cus * per_cu;
+    int per_cu_w = 0;
+    if (!hip_ok(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_w, expand_kernel<false, WAVE_T, WAVE_KW>, WAVE_T, 0),
+                "occupancy"))
+        return DQC_E_HIP;
+    h->wave_wg = cus * std::max(per_cu_w, 1);
     if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
     // non-blocking stream: every host read of its results synchronizes on it explicitly
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
@@ -2061,6 +2081,20 @@ static dqc_status size_buffers(dqc_handle* h) {
     return DQC_OK;
 }
 
+// The wavefront-per-task kernel may run: enabled, and every id fits its LDS bitmaps.
+static bool wave_ok(const dqc_handle* h) { return h->cfg.wave_tasks > 0 && !h->big && h->kloc <= WAVE_KMAX; }
+// A batch whose largest task has maxtot vertices runs one task per wavefront.
+static bool wave_batch(const dqc_handle* h, unsigned long long maxtot) {
+    return wave_ok(h) && maxtot <= (unsigned long long)h->cfg.wave_tasks;
+}
+// Device-resident steps fix the kernel when they are captured: the wavefront kernel when
+// every task of the graph (at most kloc vertices) is small enough for it.
+static bool wave_resident(const dqc_handle* h) { return wave_ok(h) && h->kloc <= h->cfg.wave_tasks; }
+// Per-workgroup scratch slots: one per workgroup of the larger of the two grids.
+static unsigned long long scratch_slots(const dqc_handle* h) {
+    return (unsigned long long)std::max(h->launch_wg, wave_ok(h) ? h->wave_wg : 0);
+}
+
 // Uploads n ids' CSR lists; kloc is the largest vertex count of one graph (n itself for
 // a single graph; batched graphs share the id space with 32-aligned bases).
 static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint64_t* out_off, const int32_t* out_nbr,
@@ -2113,7 +2147,7 @@ static dqc_status upload_impl(dqc_handle* h, int32_t n, int32_t kloc, const uint
     const unsigned long long cap_wg = (96ull << 30) / (h->stride * sizeof(uint32_t));
     h->launch_wg = (int)std::max<unsigned long long>(1, std::min<unsigned long long>(h->workgroups, cap_wg > 1 ? cap_wg - 1 : 1));
     h->st.workgroups = h->launch_wg;
-    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(h->launch_wg + 1), "scratch alloc")) return DQC_E_HIP;
+    if (!dev_alloc(h, &h->d_scratch, h->stride * (size_t)(scratch_slots(h) + 1), "scratch alloc")) return DQC_E_HIP;
     if (h->big && !hip_ok(h, hipMemset(h->d_scratch, 0, sizeof(uint32_t) * h->stride * (size_t)(h->launch_wg + 1)), "scratch clear"))
         return DQC_E_HIP;
     h->st.n_condensed = n;
@@ -2334,6 +2368,11 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
 // Most pieces a split task is cut into: dqc_config.split_pieces, or one per workgroup.
 static int max_pieces(const dqc_handle* h) { return h->cfg.split_pieces > 0 ? h->cfg.split_pieces : h->launch_wg; }
 
+// Workgroups of a full launch for the popped batch's kernel.
+static unsigned long long grid_cap(const dqc_handle* h) {
+    return (unsigned long long)(h->batch_wave ? h->wave_wg : h->launch_wg);
+}
+
 static bool batch_has_long_task(const TaskDesc* b, unsigned long long n, int split_b) {
     for (unsigned long long i = 0; i < n; ++i)
         if (task_steps(b[i]) >= long_task_steps(split_b)) return true;
@@ -2368,9 +2407,10 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         unsigned long long mt = 0;
         for (unsigned long long j = 0; j < c; ++j) mt = std::max<unsigned long long>(mt, (unsigned long long)at(j).tot);
         h->batch_maxtot = std::max<unsigned long long>(mt, 1);
+        h->batch_wave = wave_batch(h, h->batch_maxtot);
         const unsigned long long room_v = (h->cfg.spill_vertices - h->st_v - h->def_v) / h->batch_maxtot;
         const unsigned long long room_t = h->cfg.spill_tasks - h->st_t - h->def_t;
-        h->batch_grid = std::min<unsigned long long>((unsigned long long)h->launch_wg, std::min(room_v, room_t));
+        h->batch_grid = std::min<unsigned long long>(grid_cap(h), std::min(room_v, room_t));
     };
     room(cnt);
     if (h->batch_grid == 0) {
@@ -2388,7 +2428,7 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
     }
     // under stack pressure (fewer workgroups than resident), two tasks per workgroup: a
     // full batch would mostly be deferred untouched
-    if (h->batch_grid < (unsigned long long)h->launch_wg) cnt = std::min(cnt, 2 * h->batch_grid);
+    if (h->batch_grid < grid_cap(h)) cnt = std::min(cnt, 2 * h->batch_grid);
     // the batch: descriptor indices [first, first + cnt), records [v0, v0 + recs)
     const unsigned long long first = from_main ? h->st_t - cnt : lo;
     const TaskDesc* b = from_main ? rd.data() + (m - cnt) : rd.data();
@@ -2532,6 +2572,7 @@ static void fold_stats(dqc_handle* h, const GCtr& g) {
     h->st.truncated_children = (int64_t)g.truncated;
     h->st.stack_pushbacks = (int64_t)g.pushbacks;
     h->st.time_slices = (int64_t)g.slices;
+    h->st.wave_launches = h->wave_launches + (wave_resident(h) ? (int64_t)g.steps : 0);
     h->st.peak_stack_records = (int64_t)h->peak_v;
     h->st.peak_stack_tasks = (int64_t)h->peak_t;
     for (int i = 0; i < 8; ++i) h->phase[i] = g.phase[i];
@@ -2547,12 +2588,15 @@ static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long nta
     q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
     q.used = ((h->st_t + h->def_t) << CNT_SHIFT) | (h->st_v + h->def_v);
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
-    unsigned long long gmax = (unsigned long long)h->launch_wg;
+    const bool wave = !P.split_s && h->batch_wave;
+    unsigned long long gmax = (unsigned long long)(wave ? h->wave_wg : h->launch_wg);
     if (!P.split_s) gmax = std::min(gmax, h->batch_grid);
     const int grid = (int)std::min<unsigned long long>(gmax, ntask);
     hipEventRecord(h->ev0, h->stream);
     if (h->big) expand_kernel<true><<<grid, NT, 0, h->stream>>>(P);
+    else if (wave) expand_kernel<false, WAVE_T, WAVE_KW><<<grid, WAVE_T, 0, h->stream>>>(P);
     else expand_kernel<false><<<grid, NT, 0, h->stream>>>(P);
+    h->wave_launches += wave ? 1 : 0;
     hipEventRecord(h->ev1, h->stream);
     if (!hip_ok(h, hipGetLastError(), "expand launch")) return DQC_E_HIP;
     GCtr g{};
@@ -2669,8 +2713,10 @@ static dqc_status split_root(dqc_handle* h) {
 // Enqueues one device-resident step on the handle's stream: pop, (costs, order), gather,
 // expand. Nothing in it waits for the host.
 static void enqueue_step(dqc_handle* h, int32_t sched) {
+    const bool wave = wave_resident(h);
+    const int wg = wave ? h->wave_wg : h->launch_wg;
     PopParams pp{h->d_sdesc, h->d_q, h->d_g, h->cfg.pool_tasks, h->cfg.pool_vertices, h->cfg.spill_tasks,
-                 h->cfg.spill_vertices, (unsigned long long)h->launch_wg};
+                 h->cfg.spill_vertices, (unsigned long long)wg};
     pop_kernel<<<1, POP_T, 0, h->stream>>>(pp);
     const uint32_t* perm = nullptr;
     const uint32_t* mark = nullptr;
@@ -2680,7 +2726,7 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
             desc_cost_kernel<<<blocks, 256, 0, h->stream>>>(h->d_model, h->d_sdesc, 0, h->d_cost, h->d_q);
         }
         order_kernel<<<1, POP_T, 0, h->stream>>>(h->d_sdesc, h->d_q, h->d_cost, sched == DQC_SCHED_LEARNED,
-                                                 h->cfg.split_branches, h->cfg.split_chunk, max_pieces(h), h->launch_wg,
+                                                 h->cfg.split_branches, h->cfg.split_chunk, max_pieces(h), wg,
                                                  h->d_perm, h->d_mark);
         perm = h->d_perm;
         mark = h->d_mark;
@@ -2688,8 +2734,9 @@ static void enqueue_step(dqc_handle* h, int32_t sched) {
     gather_kernel<<<1024, 256, 0, h->stream>>>(h->d_sdesc, h->d_svtx, h->d_q, perm, mark, h->d_bdesc, h->d_bvtx);
     KParams P = make_params(h, ~0ull);
     P.maxtot = ~0ull;  // read from the device-resident pop
-    if (h->big) expand_kernel<true><<<h->launch_wg, NT, 0, h->stream>>>(P);
-    else expand_kernel<false><<<h->launch_wg, NT, 0, h->stream>>>(P);
+    if (h->big) expand_kernel<true><<<wg, NT, 0, h->stream>>>(P);
+    else if (wave) expand_kernel<false, WAVE_T, WAVE_KW><<<wg, WAVE_T, 0, h->stream>>>(P);
+    else expand_kernel<false><<<wg, NT, 0, h->stream>>>(P);
 }
 
 // Device-resident expansion: replays a captured HIP graph of resident_steps steps until

@@ -5,6 +5,8 @@ The knobs change only when and where work runs, never the output or the tree cou
   split_branches  least branches of a task the SIZE / LEARNED schedules may split
   split_chunk     least branches per piece
   resident_steps  0 (host-driven steps) or S steps per replay of one captured HIP graph
+  wave_tasks      the wavefront-vs-workgroup cutoff: batches whose tasks all have at most
+                  this many vertices run one task per 64-lane wavefront (-1: never)
 Every candidate runs the same complete unit of work: the whole enumeration, or the
 enumeration cut at max_depth (a deterministic prefix of the tree). The fastest wins. The
 search is coordinate descent: one knob at a time, the others held at their best so far,
@@ -23,7 +25,8 @@ import cudqc_amd as dq
 GRID = (("pool_per_wg", (8, 16, 32, 64)),
         ("split_branches", (256, 1024, 4096)),
         ("split_chunk", (16, 32, 64)),
-        ("resident_steps", (0, 16)))
+        ("resident_steps", (0, 16)),
+        ("wave_tasks", (-1, 64, 256, 2048)))
 
 
 def apply(base, knobs, workgroups):
@@ -31,7 +34,7 @@ def apply(base, knobs, workgroups):
     cfg = dq.Config.from_buffer_copy(base)
     if knobs.get("pool_per_wg") is not None:
         cfg.pool_tasks = int(knobs["pool_per_wg"]) * int(workgroups)
-    for name in ("split_branches", "split_chunk", "resident_steps"):
+    for name in ("split_branches", "split_chunk", "resident_steps", "wave_tasks"):
         if knobs.get(name) is not None:
             setattr(cfg, name, int(knobs[name]))
     return cfg

@@ -65,14 +65,20 @@ typedef struct {
                                   as a launch has workgroups. A batch with several
                                   tasks to split gives each about 4 x workgroups /
                                   (their number) pieces, so the batch fills a launch  */
-    int32_t root_first;        /* the children of level-0 steps s < root_first are
-                                  counted in truncated_children, not expanded (the
-                                  branches still run, so the host level-1 child is the
-                                  reference's); 0 = none                              */
+    int32_t root_first;        /* > 0: the unit starts at level-0 step root_first. The
+                                  steps before it run only until the first one that
+                                  writes a child (that child keeps the host level-1
+                                  role, as in the reference); those branches emit no
+                                  clique and count in no statistic; 0 = from step 0   */
     int32_t launch_ms;         /* > 0: a task still running after launch_ms in its
                                   launch pushes the rest of its steps back onto the
                                   stack as a piece and the launch ends soon after
                                   (output unchanged); 0 = launches run to completion  */
+    int32_t wave_tasks;        /* > 0: a batch whose tasks all have at most wave_tasks
+                                  vertices runs one task per 64-lane wavefront instead
+                                  of one per 256-thread workgroup (condensed graphs of
+                                  at most 2048 vertices; output identical); 0 = the
+                                  default (DQC_WAVE_TASKS_DEFAULT), negative = never  */
 } dqc_config;
 
 typedef struct {
@@ -97,6 +103,7 @@ typedef struct {
     double rmnonmax_ms;        /* search entry points: the maximal-set filter         */
     int64_t time_slices;       /* tasks whose rest went back onto the stack at the end
                                   of a launch_ms time slice                           */
+    int64_t wave_launches;     /* expansion launches that ran one task per wavefront  */
 } dqc_stats;
 
 /* Receives each maximal set (ascending ids, sets in the DQC-R file's order). */
@@ -155,6 +162,9 @@ dqc_status dqc_push_root(dqc_handle* h, const uint32_t* records, int32_t nvert, 
  * reference src/host_functions.cu:92-285, the d_expand_level / d_transfer_buffers /
  * d_fill_from_buffer kernels and the cuTS MPI stealing). sched is one of DQC_SCHED_*;
  * the enumeration output and the tree counts do not depend on it. */
+/* dqc_config.wave_tasks when left 0: batches of tasks of at most this many vertices run
+ * one task per wavefront */
+#define DQC_WAVE_TASKS_DEFAULT 256
 #define DQC_SCHED_DEFAULT 0 /* the handle's dqc_config.schedule                      */
 #define DQC_SCHED_FIFO 1    /* plain: batches popped from the top of the LIFO task stack,
                                expanded in stack order, never split                   */

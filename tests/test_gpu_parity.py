@@ -22,17 +22,27 @@ def run_case(dqc, c, tmp_path, config=None):
     return cnt, st, mx
 
 
-def test_golden_cases_bit_exact(dqc, manifest, tmp_path):
+# wave_tasks: 0 = the library default (tasks of up to 256 vertices one per wavefront),
+# -1 = every task on a 256-thread workgroup, 2048 = every batch that fits the wavefront
+# kernel (graphs of at most 2048 condensed vertices) on it
+@pytest.mark.parametrize("wave_tasks", [0, -1, 2048])
+def test_golden_cases_bit_exact(dqc, manifest, tmp_path, wave_tasks):
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.wave_tasks = wave_tasks
     bad = []
+    waves = 0
     for c in manifest["cases"]:
-        print("case", c["name"], flush=True)
-        cnt, st, mx = run_case(dqc, c, tmp_path)
+        print("case", c["name"], "wave_tasks", wave_tasks, flush=True)
+        cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
+        waves += st.wave_launches
         got = open(mx, "rb").read() if os.path.exists(mx) else b""
         exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
         if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
                 or st.raw_cliques != c["raw_cliques"]):
             bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"]))
     assert not bad, bad
+    assert (waves == 0) == (wave_tasks < 0), waves
 
 
 def test_small_batches_same_result(dqc, manifest, tmp_path):

@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 GRID = (("pool_per_wg", (1, 32)),
         ("split_branches", (2, 1024)),
         ("split_chunk", (3, 32)),
-        ("resident_steps", (0, 4)))
+        ("resident_steps", (0, 4)),
+        ("wave_tasks", (-1, 2048)))
 
 
 def test_tuner_keeps_counts_and_output(dqc, manifest, tmp_path):
