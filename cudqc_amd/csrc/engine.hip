@@ -75,6 +75,7 @@ enum ErrBits : unsigned { ERR_SPILL = 1u, ERR_CLIQUES = 2u, ERR_MEMBERS = 4u, ER
 constexpr int32_t TF_PIECE = 1;         // a piece of a split task: not counted, not split again
 constexpr int32_t TF_REMOVE_FIRST = 2;  // the piece's first step starts with a removal
 constexpr int32_t TF_SPLIT = 4;         // set by the batch scheduler: cut this task into pieces
+constexpr int32_t TF_L1_DONE = 8;       // a level-0 piece whose task already wrote its first child
 
 struct TaskDesc {
     unsigned long long off;
@@ -85,6 +86,7 @@ struct TaskDesc {
     uint32_t uid, parent;
     uint16_t depth, gid;  // gid: graph of a batched run (0 for a single graph)
     uint16_t dmin_o, dmax_o, dmin_i, dmax_i;  // min / max of out / in degree inside the task
+    int32_t step0;  // a piece: loop steps of its task before the piece's first (level-0 windows)
 };
 static_assert(sizeof(TaskDesc) == 56, "task descriptor layout");
 
@@ -94,14 +96,15 @@ __host__ __device__ inline bool split_eligible(const TaskDesc& d, int split_b) {
     const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
     return split_b > 0 && !(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b;
 }
-// Most pieces per split task in a batch with n_split eligible tasks: enough for about
-// four pieces per workgroup over the batch (one long task gets one piece per workgroup,
-// many share them), and at most max_pieces.
-__host__ __device__ inline int batch_piece_cap(int max_pieces, int workgroups, unsigned long long n_split) {
-    if (n_split == 0) return max_pieces;
-    const unsigned long long want = (4ull * (unsigned long long)workgroups + n_split - 1) / n_split;
-    const unsigned long long cap = want < 2 ? 2 : want;
-    return (int)(cap < (unsigned long long)max_pieces ? cap : (unsigned long long)max_pieces);
+// Pieces for a split task of estimated cost c in a batch of total estimated cost C: the
+// batch's work cut into about four pieces per workgroup of similar cost (C / (4 x
+// workgroups) each), so a task gets pieces in proportion to its cost; at least 2, at most
+// max_pieces.
+__host__ __device__ inline int pieces_for(double c, double total, int workgroups, int max_pieces) {
+    const double target = total / (4.0 * (double)(workgroups > 0 ? workgroups : 1));
+    double p = target > 0.0 ? ceil(c / target) : (double)max_pieces;
+    p = fmin(fmax(p, 2.0), (double)(max_pieces > 2 ? max_pieces : 2));
+    return (int)p;
 }
 // Steps per piece when a task of `steps` loop steps is split: as many pieces as
 // max_pieces allows, each of at least split_c steps (host and device schedulers both
@@ -237,12 +240,17 @@ struct Field {
     __device__ __forceinline__ uint32_t& operator[](long long x) const { return p[4 * x]; }
 };
 
+// The kernel's parameter block, read in place from the kernarg segment (constant
+// address space: scalar loads). Taking the address of the by-value kernel argument
+// instead would copy the whole block into per-lane scratch memory.
+typedef const __attribute__((address_space(4))) KParams* KPtr;
+
 // All functions below are executed by every thread of the workgroup with uniform
 // control flow; `W` bundles the per-workgroup pointers and uniform scalars. NT threads
 // run one task: a 256-thread workgroup, or one wavefront (64) for small tasks.
 template <int NT>
 struct W {
-    const KParams* P;
+    KPtr P;
     Shared* S;
     int tid;
     int kw;  // bitmap words in use
@@ -898,7 +906,7 @@ struct W {
         if (tid == 0) {
             TaskDesc d;
             d.off = o; d.tot = N; d.nm = cm; d.ncov = 0; d.mode = mode;
-            d.iters = N - cm; d.flags = 0;
+            d.iters = N - cm; d.flags = 0; d.step0 = 0;
             d.uid = (uint32_t)(atomicAdd(&P->g->next_uid, 1ull) + 1);
             d.parent = cur_uid;
             d.depth = (uint16_t)min(cur_depth + 1u, 65535u); d.gid = (uint16_t)cur_gid;
@@ -973,8 +981,10 @@ struct W {
     // current degrees and lvl2adj) as a piece that runs `iters` loop steps from here.
     // deferred: into the deferred region (backpressure), which is expanded only when the
     // main stack is empty, after the children written below the task have run.
+    // step0: loop steps of the task before the piece's first; l1_done: a level-0 task whose
+    // first child (the host level-1 task) is already written.
     __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters,
-                                                bool deferred = false) {
+                                                bool deferred, int step0, bool l1_done) {
         unsigned long long a = 0;
         if (tid == 0) a = reserve(deferred, (unsigned long long)tot);
         a = bcast0(a);
@@ -992,7 +1002,8 @@ struct W {
             TaskDesc d = src;
             d.off = o; d.tot = tot; d.nm = nm; d.ncov = 0; d.mode = mode;
             d.iters = iters;
-            d.flags = TF_PIECE | (remove_first ? TF_REMOVE_FIRST : 0);
+            d.flags = TF_PIECE | (remove_first ? TF_REMOVE_FIRST : 0) | (l1_done ? TF_L1_DONE : 0);
+            d.step0 = step0;
             P->sp_desc[t] = d;
         }
         bytes += 24ull * tot / NT + 1;
@@ -1339,13 +1350,16 @@ struct W {
         __syncthreads();
         int tot = T, nc = T - nm;
         int expansions = piece ? min(d.iters, nc) : nc;
-        if (mode == MODE_HOST_L0 && P->root_steps > 0) expansions = min(expansions, ncov + P->root_first + P->root_steps);
+        // a level-0 piece resumes its task's loop at step step0 (window bounds count from there)
+        const int step0 = piece ? d.step0 : 0;
+        if (mode == MODE_HOST_L0 && P->root_steps > 0)
+            expansions = min(expansions, ncov + max(0, P->root_first + P->root_steps - step0));
         // the batch scheduler marks the tasks to split (host side, within a record budget)
         // a task marked TF_SPLIT carries its piece length (steps per piece) in iters
         const bool split = !piece && mode != MODE_HOST_L0 && (d.flags & TF_SPLIT);
         const int chunk = split ? max(1, d.iters) : 1;
         const bool remove_first = piece && (d.flags & TF_REMOVE_FIRST);
-        bool first_host_child = (mode == MODE_HOST_L0);
+        bool first_host_child = (mode == MODE_HOST_L0) && !(d.flags & TF_L1_DONE);
         for (int j = ncov; j < expansions; ++j) {
             // The split pass writes only at chunk starts, so it checks the stack only there:
             // a deferred piece then takes the place of that chunk's piece and no step is
@@ -1355,11 +1369,12 @@ struct W {
             if (low) {
                 // backpressure: the rest of the task becomes a deferred piece; at the end of
                 // a time slice it goes back onto the main stack, to run in the next launch
-                write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, low == 1);
+                write_piece(d, tot, nm, mode, j != ncov || remove_first, expansions - j, low == 1, step0 + (j - ncov),
+                            mode == MODE_HOST_L0 && !first_host_child);
                 if (tid == 0) atomicAdd(low == 1 ? &P->g->pushbacks : &P->g->slices, 1ull);
                 break;
             }
-            if (split && chunk_start) write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j));
+            if (split && chunk_start) write_piece(d, tot, nm, mode, j != ncov, min(chunk, expansions - j), false, j - ncov, false);
             if (j != ncov || remove_first) {
                 tick(0);
                 int r = remove_one(tot, nc, nm);
@@ -1377,7 +1392,7 @@ struct W {
             if (split) continue;
             // before a level-0 window only the first child matters (it takes the host
             // level-1 role); once it is found the branches there are skipped
-            const bool before_window = mode == MODE_HOST_L0 && j - ncov < P->root_first;
+            const bool before_window = mode == MODE_HOST_L0 && step0 + (j - ncov) < P->root_first;
             if (before_window && !first_host_child) continue;
             discard_child = before_window;
             bool wrote = branch(tot, nm, mode, first_host_child);
@@ -1427,7 +1442,7 @@ void expand_kernel(KParams P) {
     if (P.in_ntask == ~0ull && blockIdx.x >= P.q->wg_limit) return;  // no stack room for them
     __shared__ Shared S;
     W<TPB> w;
-    w.P = &P;
+    w.P = (KPtr)__builtin_amdgcn_kernarg_segment_ptr();  // the explicit arguments start the segment
     w.S = &S;
     w.tid = threadIdx.x;
     w.kw = (P.kloc + 31) >> 5;
@@ -1620,7 +1635,8 @@ __host__ __device__ inline int long_task_steps(int split_b) { return split_b > 0
 // x size, the prediction already in pred[]), keeping stack order within a bucket, and in
 // that order the eligible tasks are marked for splitting while the running total of
 // their piece records fits q->split_budget (a prefix rule: the first task that does not
-// fit ends the marking, as in the host pop). A batch without a task of long_task_steps
+// fit ends the marking, as in the host pop). A marked task is cut into pieces_for(its
+// cost, the batch's total cost) pieces. A batch without a task of long_task_steps
 // steps keeps stack order and splits nothing. perm[k] = the batch index of the k-th
 // task to run; split[k] = its piece length when it is to be split, else 0.
 __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict__ sdesc, const QCtr* __restrict__ q,
@@ -1632,33 +1648,39 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
     __shared__ unsigned long long scan[POP_T];
     __shared__ unsigned long long carry;
     __shared__ int any_long;
-    __shared__ unsigned n_split;
+    __shared__ double csum[POP_T];
     const unsigned long long n = q->ntask, first = q->first, budget = q->split_budget;
     const int tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
-    if (tid == 0) { carry = 0; any_long = 0; n_split = 0; }
+    if (tid == 0) { carry = 0; any_long = 0; }
     __syncthreads();
-    auto bucket = [&](unsigned long long i) {
+    auto cost = [&](unsigned long long i) {
         const TaskDesc d = sdesc[first + i];
         const double steps = (double)max(1, d.tot - d.nm);
-        double c;
         if (learned) {  // predicted branches x task size
             const double share = (d.flags & TF_PIECE) ? fmin(1.0, (double)d.iters / steps) : 1.0;
-            c = expm1((double)fminf(pred[i], 60.f)) * share * (double)d.tot;
-        } else {
-            c = ((d.flags & TF_PIECE) ? (double)d.iters : steps) * (double)d.tot;
+            return fmax(expm1((double)fminf(pred[i], 60.f)) * share * (double)d.tot, 0.0);
         }
-        int b = (int)(2.0 * log2(1.0 + fmax(c, 0.0)));
+        return ((d.flags & TF_PIECE) ? (double)d.iters : steps) * (double)d.tot;
+    };
+    auto bucket = [&](unsigned long long i) {
+        int b = (int)(2.0 * log2(1.0 + cost(i)));
         return 63 - min(max(b, 0), 63);  // bucket 0 = costliest
     };
+    double part = 0.0;
     for (unsigned long long i = tid; i < n; i += POP_T) {
         atomicAdd(&hist[bucket(i)], 1u);
+        part += cost(i);
         const TaskDesc d = sdesc[first + i];
         if (task_steps(d) >= long_task_steps(split_b)) any_long = 1;
-        if (split_eligible(d, split_b)) atomicAdd(&n_split, 1u);
     }
+    csum[tid] = part;
     __syncthreads();
-    const int piece_cap = batch_piece_cap(max_pieces, workgroups, n_split);
+    for (int off = POP_T / 2; off > 0; off >>= 1) {
+        if (tid < off) csum[tid] += csum[tid + off];
+        __syncthreads();
+    }
+    const double total = csum[0];
     if (!any_long) {  // no task long enough to split or to unbalance the launch: stack order
         for (unsigned long long i = tid; i < n; i += POP_T) { perm[i] = (uint32_t)i; split[i] = 0; }
         return;
@@ -1709,7 +1731,7 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
             const TaskDesc d = sdesc[first + perm[k]];
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (split_eligible(d, split_b)) {
-                chunk = split_chunk_for(steps, split_c, piece_cap);
+                chunk = split_chunk_for(steps, split_c, pieces_for(cost(perm[k]), total, workgroups, max_pieces));
                 need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
             }
         }
@@ -2448,15 +2470,15 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         h->hbatch.resize(cnt);
         const unsigned long long used = h->st_v + h->def_v;
         unsigned long long budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 4 : 0;
-        unsigned long long n_split = 0;
-        for (unsigned long long i = 0; i < cnt; ++i) n_split += split_eligible(b[i], h->cfg.split_branches);
-        const int piece_cap = batch_piece_cap(max_pieces(h), h->launch_wg, n_split);
+        double total = 0.0;
+        for (unsigned long long i = 0; i < cnt; ++i) total += h->hcost[i];
         for (unsigned long long i = 0; i < cnt; ++i) {
             TaskDesc d = b[order[i]];
             d.off -= v0;
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (split_eligible(d, h->cfg.split_branches)) {
-                const int chunk = split_chunk_for(steps, h->cfg.split_chunk, piece_cap);
+                const int pieces = pieces_for(h->hcost[order[i]], total, h->launch_wg, max_pieces(h));
+                const int chunk = split_chunk_for(steps, h->cfg.split_chunk, pieces);
                 const unsigned long long need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
                 if (need <= budget) {
                     budget -= need;
@@ -2506,7 +2528,7 @@ static dqc_status batch_costs(dqc_handle* h, const TaskDesc* b, size_t n, unsign
         for (size_t i = 0; i < n; ++i) {  // predicted branches x task size
             const double steps = (double)std::max(1, b[i].tot - b[i].nm);
             const double share = (b[i].flags & TF_PIECE) ? std::min(1.0, (double)b[i].iters / steps) : 1.0;
-            cost[i] = std::expm1((double)std::min(h->hpred[i], 60.f)) * share * (double)b[i].tot;
+            cost[i] = std::max(0.0, std::expm1((double)std::min(h->hpred[i], 60.f)) * share * (double)b[i].tot);
         }
         return DQC_OK;
     }

@@ -83,7 +83,8 @@ static_assert(sizeof(dqc_task_desc) == sizeof(TaskDesc) && offsetof(dqc_task_des
                   offsetof(dqc_task_desc, depth) == offsetof(TaskDesc, depth) &&
                   offsetof(dqc_task_desc, graph) == offsetof(TaskDesc, gid) &&
                   offsetof(dqc_task_desc, dmin_out) == offsetof(TaskDesc, dmin_o) &&
-                  offsetof(dqc_task_desc, dmax_in) == offsetof(TaskDesc, dmax_i),
+                  offsetof(dqc_task_desc, dmax_in) == offsetof(TaskDesc, dmax_i) &&
+                  offsetof(dqc_task_desc, first_step) == offsetof(TaskDesc, step0),
               "dqc_task_desc mirrors the kernels' task descriptor");
 
 extern "C" dqc_status dqc_expand_noalloc(const dqc_device_graph* graph, const dqc_task_desc* tasks,

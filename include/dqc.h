@@ -360,14 +360,17 @@ int32_t dqc_generate(int32_t config, int32_t graph_index, const char* path);
 /* Task descriptor in device memory (56 bytes): records [offset, offset + size) of the
  * record array, members first, then `covered` covered vertices (level-0 tasks), then
  * candidates; mode 0 / 1 / 2 as dqc_push_task; steps = size - members - covered for a
- * whole task (flags 0); flags & 1 marks a piece (a deferred rest of a task, as this call
- * writes them: it runs `steps` loop steps from its state, flags & 2 = its first step
- * removes). id, parent, depth, graph and the degree extremes are carried along. */
+ * whole task (flags 0, first_step 0); flags & 1 marks a piece (a deferred rest of a
+ * task, as this call writes them: it runs `steps` loop steps from its state, flags & 2 =
+ * its first step removes, flags & 8 = a level-0 piece whose task already wrote its first
+ * child; first_step = the task's loop steps before the piece). id, parent, depth, graph
+ * and the degree extremes are carried along. */
 typedef struct {
     uint64_t offset;
     int32_t size, members, covered, mode, steps, flags;
     uint32_t id, parent;
     uint16_t depth, graph, dmin_out, dmax_out, dmin_in, dmax_in;
+    int32_t first_step;
 } dqc_task_desc;
 
 /* A condensed graph and its tables in device memory (as dqc_upload_graph and

@@ -57,7 +57,8 @@ class TaskDesc(ctypes.Structure):
                 ("covered", ctypes.c_int32), ("mode", ctypes.c_int32), ("steps", ctypes.c_int32),
                 ("flags", ctypes.c_int32), ("id", ctypes.c_uint32), ("parent", ctypes.c_uint32),
                 ("depth", ctypes.c_uint16), ("graph", ctypes.c_uint16), ("dmin_out", ctypes.c_uint16),
-                ("dmax_out", ctypes.c_uint16), ("dmin_in", ctypes.c_uint16), ("dmax_in", ctypes.c_uint16)]
+                ("dmax_out", ctypes.c_uint16), ("dmin_in", ctypes.c_uint16), ("dmax_in", ctypes.c_uint16),
+                ("first_step", ctypes.c_int32)]
 
 
 class DeviceGraph(ctypes.Structure):
