@@ -56,6 +56,10 @@ constexpr int HMAX = 256;          // member-degree histogram bins (members < HM
 constexpr int WAVE_T = 64;
 constexpr int WAVE_KW = 64;
 constexpr int WAVE_KMAX = WAVE_KW * 32;
+// Split passes (the removal and lookahead steps of a task cut into pieces, one step after
+// another) run on workgroups of SPLIT_T threads: each step scans the removed vertex's
+// lists, and four times the lanes cut the memory round trips of a long list fourfold.
+constexpr int SPLIT_T = 1024;
 #ifndef DQC_SCAN_UNR
 #define DQC_SCAN_UNR 8
 #endif
@@ -1867,6 +1871,9 @@ struct dqc_handle {
     bool big = false;    // condensed graph above KMAX vertices: HBM id bitmaps
     int launch_wg = 0;   // workgroups per launch (the scratch bounds it for large graphs)
     int wave_wg = 0;     // resident single-wavefront workgroups of the wavefront-per-task kernel
+    int split_wg = 0;    // resident workgroups of the split-pass kernel
+    unsigned long long batch_nsplit = 0;  // the popped batch's first batch_nsplit tasks are split passes
+    unsigned long long batch_pend_rest = 0;  // (tasks << 34) | records of the batch after them
     bool batch_wave = false;  // the popped batch runs one task per wavefront
     int64_t wave_launches = 0;
     unsigned long long *d_oo = nullptr, *d_io = nullptr, *d_to = nullptr;
@@ -2033,6 +2040,11 @@ dqc_status dqc_open(const dqc_config* cfg, dqc_handle** out) {
                 "occupancy"))
         return DQC_E_HIP;
     h->wave_wg = cus * std::max(per_cu_w, 1);
+    int per_cu_s = 0;
+    if (!hip_ok(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, expand_kernel<true, SPLIT_T>, SPLIT_T, 0),
+                "occupancy"))
+        return DQC_E_HIP;
+    h->split_wg = cus * std::max(per_cu_s, 1);
     if (!c.pool_tasks) c.pool_tasks = 32ull * (unsigned long long)h->workgroups;
     // non-blocking stream: every host read of its results synchronizes on it explicitly
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate")) return DQC_E_HIP;
@@ -2456,6 +2468,8 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
     const TaskDesc* b = from_main ? rd.data() + (m - cnt) : rd.data();
     const unsigned long long v0 = b[0].off, recs = b[cnt - 1].off + (unsigned long long)b[cnt - 1].tot - v0;
     h->batch_pend = (cnt << CNT_SHIFT) | recs;
+    h->batch_nsplit = 0;
+    h->batch_pend_rest = h->batch_pend;
     if (sched != DQC_SCHED_FIFO && !batch_has_long_task(b, cnt, h->cfg.split_branches))
         sched = DQC_SCHED_FIFO;  // nothing to split or to balance: stack order keeps locality
     if (sched == DQC_SCHED_FIFO) {
@@ -2490,6 +2504,14 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
             }
             h->hbatch[i] = d;
         }
+        // the tasks to split run first, in a launch of their own on the split-pass kernel
+        std::stable_partition(h->hbatch.begin(), h->hbatch.end(), [](const TaskDesc& d) { return (d.flags & TF_SPLIT) != 0; });
+        unsigned long long rest_v = 0;
+        for (unsigned long long i = 0; i < cnt; ++i) {
+            if (h->hbatch[i].flags & TF_SPLIT) ++h->batch_nsplit;
+            else rest_v += (unsigned long long)h->hbatch[i].tot;
+        }
+        h->batch_pend_rest = ((cnt - h->batch_nsplit) << CNT_SHIFT) | rest_v;
         if (!hip_ok(h, hipMemcpyAsync(h->d_bdesc, h->hbatch.data(), sizeof(TaskDesc) * cnt, hipMemcpyHostToDevice, h->stream),
                     "batch upload"))
             return DQC_E_HIP;
@@ -2601,21 +2623,27 @@ static void fold_stats(dqc_handle* h, const GCtr& g) {
 }
 
 // Launches one expansion over `ntask` work items and folds its counters into the stats.
-static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long ntask) {
+// split_pass: the work items are tasks to split (the split-pass kernel runs them), and
+// pend = the batch's tasks not yet finished.
+static dqc_status launch(dqc_handle* h, const KParams& P, unsigned long long ntask, bool split_pass = false,
+                         unsigned long long pend = ~0ull) {
     QCtr q{};
     q.next_task = 0;
     q.sp_alloc = (h->st_t << CNT_SHIFT) | h->st_v;
     q.root_first = ~0ull;
-    q.pend = P.split_s ? 0 : h->batch_pend;
+    q.pend = P.split_s ? 0 : (pend == ~0ull ? h->batch_pend : pend);
     q.def_alloc = (h->def_t << CNT_SHIFT) | h->def_v;
     q.used = ((h->st_t + h->def_t) << CNT_SHIFT) | (h->st_v + h->def_v);
     if (!hip_ok(h, hipMemcpyAsync(h->d_q, &q, sizeof q, hipMemcpyHostToDevice, h->stream), "ctr write")) return DQC_E_HIP;
-    const bool wave = !P.split_s && h->batch_wave;
-    unsigned long long gmax = (unsigned long long)(wave ? h->wave_wg : h->launch_wg);
+    const bool wave = !P.split_s && !split_pass && h->batch_wave;
+    unsigned long long gmax = (unsigned long long)(wave ? h->wave_wg : split_pass ? std::min(h->split_wg, h->launch_wg)
+                                                                                  : h->launch_wg);
     if (!P.split_s) gmax = std::min(gmax, h->batch_grid);
     const int grid = (int)std::min<unsigned long long>(gmax, ntask);
     hipEventRecord(h->ev0, h->stream);
-    if (h->big) expand_kernel<true><<<grid, NT, 0, h->stream>>>(P);
+    if (split_pass && h->big) expand_kernel<true, SPLIT_T><<<grid, SPLIT_T, 0, h->stream>>>(P);
+    else if (split_pass) expand_kernel<false, SPLIT_T><<<grid, SPLIT_T, 0, h->stream>>>(P);
+    else if (h->big) expand_kernel<true><<<grid, NT, 0, h->stream>>>(P);
     else if (wave) expand_kernel<false, WAVE_T, WAVE_KW><<<grid, WAVE_T, 0, h->stream>>>(P);
     else expand_kernel<false><<<grid, NT, 0, h->stream>>>(P);
     h->wave_launches += wave ? 1 : 0;
@@ -2844,8 +2872,17 @@ static dqc_status run_launches(dqc_handle* h, int32_t sched, int64_t max_launche
             unsigned long long m = 0;
             dqc_status rc = pop_batch(h, sched, &m);
             if (rc != DQC_OK) return rc;
-            rc = launch(h, make_params(h, m), m);
-            if (rc != DQC_OK) return rc;
+            const unsigned long long ns = h->batch_nsplit;
+            if (ns > 0) {  // the split passes first, then the rest of the batch
+                rc = launch(h, make_params(h, ns), ns, true);
+                if (rc != DQC_OK) return rc;
+            }
+            if (m > ns) {
+                KParams P = make_params(h, m - ns);
+                P.in_desc = h->d_bdesc + ns;
+                rc = launch(h, P, m - ns, false, h->batch_pend_rest);
+                if (rc != DQC_OK) return rc;
+            }
         }
         if (progress) {
             auto now = std::chrono::steady_clock::now();
