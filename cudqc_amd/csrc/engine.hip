@@ -244,6 +244,10 @@ struct Field {
     __device__ __forceinline__ uint32_t& operator[](long long x) const { return p[4 * x]; }
 };
 
+// A few zero words in device memory: the branch-free list scans load a bitmap word
+// from here instead when the live-set summary rules an id out.
+__device__ uint32_t zero_words[16];
+
 // The kernel's parameter block, read in place from the kernarg segment (constant
 // address space: scalar loads). Taking the address of the by-value kernel argument
 // instead would copy the whole block into per-lane scratch memory.
@@ -271,6 +275,7 @@ struct W {
     // HBM read: the common case, since few neighbours of a vertex are in the task.
     uint32_t* lsum;  // null in the LDS-bitmap variant
     int gsh;
+    const uint32_t* zw;  // zero_words
     __device__ __forceinline__ bool maybe_live(int id) const {
         const int g = (id - gb) >> gsh;
         return (lsum[g >> 5] >> (g & 31)) & 1u;
@@ -346,6 +351,61 @@ struct W {
         return m ? bm[(id - gb) >> 5] : 0u;
     }
     __device__ __forceinline__ static uint32_t bbit(uint32_t word, int id) { return (word >> (id & 31)) & 1u; }
+    // Address of id's bitmap word, or of a zero word when the live-set summary rules the id
+    // out: every lane issues its load, so a batch of them is counted exactly by the load
+    // counters and the next batch of list entries can stay in flight meanwhile.
+    __device__ __forceinline__ const uint32_t* bword_ptr(const uint32_t* bm, int id) const {
+        const uint32_t* p = bm + ((id - gb) >> 5);
+        return (!lsum || maybe_live(id)) ? p : zw;
+    }
+
+    // Appends to out[] (counted in S->cnt[slot], which the caller zeroed before a barrier)
+    // every entry x of the list a[b, e) whose bmA bit is set and bmB bit clear, x != skip.
+    // Software-pipelined and branch-free: a round's bitmap words are requested, then the
+    // next round's list entries, then the words are tested, so the next round's entries
+    // are in flight meanwhile; a wavefront appends its hits with one LDS atomic per
+    // round and unrolled step. The order of out[] is not defined.
+    __device__ __forceinline__ void wg_collect(const int32_t* __restrict__ a, unsigned long long b,
+                                               unsigned long long e, const uint32_t* bmA, const uint32_t* bmB,
+                                               int skip, int32_t* out, int slot) const {
+        if (e <= b) return;
+        const int32_t* ap = a + b;
+        const int n = (int)(e - b);
+        const int last = n - 1;
+        constexpr int U = SCAN_UNR;
+        const int lane = tid & 63;
+        int k = tid;
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = ap[min(k + u * NT, last)];
+        while (k < n) {
+            uint32_t wa[U], wb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                wa[u] = *bword_ptr(bmA, v[u]);
+                wb[u] = *bword_ptr(bmB, v[u]);
+            }
+            const int k2 = k + U * NT;
+            int v2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v2[u] = ap[min(k2 + u * NT, last)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int x = v[u];
+                const bool hit = k + u * NT < n && bbit(wa[u], x) && !bbit(wb[u], x) && x != skip;
+                const unsigned long long m = __ballot(hit);
+                if (m) {
+                    int base = 0;
+                    if (lane == __builtin_ctzll(m)) base = atomicAdd(&S->cnt[slot], __popcll(m));
+                    base = __shfl(base, __builtin_ctzll(m), 64);
+                    if (hit) out[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = v2[u];
+            k = k2;
+        }
+    }
 
     // Visits every entry of the list a[b, e) once, spread over the workgroup, as
     // f(id, bit in bmA, bit in bmB) (bmB may be null: bit 0). Each thread issues SCAN_UNR
@@ -1044,13 +1104,20 @@ struct W {
                 if (pom[w] + oc < mo) { if (memb) fail = 1; else if (w == last) skip = 1; }
             }
         });
-        wg_scan_bits(P->tn, P->to[p], P->to[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
-            if (live && !memb) pl2[w] -= 1u;
-        });
+        // two-hop list: the live non-members in it (collected into gat) lose one lvl2adj
+        wg_collect(P->tn, P->to[p], P->to[p + 1], plive, pmem, -1, gat, 9);
         if (tid == 0)
             bytes += 4ull * ((P->oo[p + 1] - P->oo[p]) + (P->io[p + 1] - P->io[p]) + (P->to[p + 1] - P->to[p])) + 48;
-        if (__syncthreads_or(fail)) return 0;
-        return __syncthreads_or(skip) ? 2 : 1;
+        const int bad = __syncthreads_or(fail) ? 1 : __syncthreads_or(skip) ? 2 : 0;  // barriers: gat complete
+        const int nh = S->cnt[9];
+        for (int i = tid; i < nh; i += NT) {
+            const int w = gat[i];
+            pl2[w] -= 1u;
+        }
+        __syncthreads();
+        if (tid == 0) S->cnt[9] = 0;
+        if (bad == 1) return 0;
+        return bad == 2 ? 2 : 1;
     }
 
     // reference src/host_functions.cu:1333-1383 (host: lvl2adj >= tot - 1) and
@@ -1090,17 +1157,18 @@ struct W {
             com[x] = pom[x]; coc[x] = poc[x]; cim[x] = pim[x]; cic[x] = pic[x];
             bs(cmem, x);
         }
-        // candidates within two hops of v (the others fail diameter pruning)
-        wg_scan_bits(P->tn, P->to[v], P->to[v + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
-            if (live && !memb && w != v) {
-                com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
-                bs(cL, w);
-                lst[atomicAdd(&S->cnt[3], 1)] = w;
-            }
-        });
+        // candidates within two hops of v (the others fail diameter pruning): collected
+        // into lst, then their degrees copied
+        wg_collect(P->tn, P->to[v], P->to[v + 1], plive, pmem, v, lst, 3);
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
         __syncthreads();
         const int nl = S->cnt[3];
+        for (int i = tid; i < nl; i += NT) {
+            const int w = lst[i];
+            com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
+            bs(cL, w);
+        }
+        __syncthreads();
         dirty_l = nl;
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
         wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], cmem, cL, [&](int w, uint32_t m, uint32_t l) {
@@ -1447,6 +1515,7 @@ void expand_kernel(KParams P) {
     __shared__ Shared S;
     W<TPB> w;
     w.P = (KPtr)__builtin_amdgcn_kernarg_segment_ptr();  // the explicit arguments start the segment
+    w.zw = zero_words;
     w.S = &S;
     w.tid = threadIdx.x;
     w.kw = (P.kloc + 31) >> 5;
@@ -1484,7 +1553,7 @@ void expand_kernel(KParams P) {
     w.launch_steps = 0;
     w.t_start = __builtin_amdgcn_s_memrealtime();
     w.discard_child = false;
-    if (threadIdx.x == 0) S.acc_bytes = 0;
+    if (threadIdx.x == 0) { S.acc_bytes = 0; S.cnt[9] = 0; }
     __syncthreads();
     while (true) {
         unsigned long long t = 0;
