@@ -40,9 +40,13 @@ R0, B, DEPTH = (int(x) for x in os.environ.get("DQC_UNIT", "90112,64,2").split("
 WORKLOAD_NAME = ("R-MAT 2^20 directed, 2^24 arcs (.57/.19/.19/.05) + 500 planted QCs of 20-40 @0.95; "
                  "gamma_out=gamma_in=0.9, min_size=18; unit: level-0 steps [%d, %d) and their subtrees to depth %d"
                  % (R0, R0 + B, DEPTH))
-# HBM traffic of expand_kernel per timed step of this command, from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE passes scaled by a copy calibration (profiles/), or None
-PMC_TRAFFIC_PER_STEP = None
+# HBM-side traffic of expand_kernel per step of this unit (bytes), from two rocprofv3
+# --pmc passes over this command (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
+# 256 MiB copy reads on the same counters); profiles/expand_kernel_rmat20_unit.md. It is a
+# measurement of this build, not of the run that prints it.
+PMC_TRAFFIC_PER_STEP = 1018.6e9
+PMC_TRAFFIC_SOURCE = ("profiles/expand_kernel_rmat20_unit.md: rocprofv3 --pmc FETCH_SIZE (x2.0) and "
+                      "WRITE_SIZE (x1.0) passes over this unit; 731.3 GB fetched + 287.3 GB written per step")
 
 
 def unit_config(dq, sched):
@@ -57,16 +61,16 @@ def unit_config(dq, sched):
 
 def run_unit(eng, root, ncov, sched):
     """One complete expansion of the unit from the resident level-0 task; returns
-    (wall seconds, stats delta dict)."""
+    (wall seconds, stats, kernel ms, launches) of that run."""
     eng.reset()
     eng.push_root(root, ncov)
     s0 = eng.stats()
-    k0 = s0.expand_kernel_ms
+    k0, l0 = s0.expand_kernel_ms, s0.launches
     t = time.perf_counter()
     eng.run(sched)
     wall = time.perf_counter() - t
     st = eng.stats()
-    return wall, st, st.expand_kernel_ms - k0
+    return wall, st, st.expand_kernel_ms - k0, st.launches - l0
 
 
 def maximal_unit(dq, eng, tmp):
@@ -173,7 +177,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-tasks", type=int, default=16, help="unit tasks the CPU baseline expands")
+    ap.add_argument("--cpu-tasks", type=int, default=64, help="unit tasks the CPU baseline expands (64: all of them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-schedules", action="store_true", help="skip the FIFO / SIZE runs of the unit")
     ap.add_argument("--no-complete", action="store_true", help="skip the completed workloads (ER 1000, batch)")
@@ -217,7 +221,7 @@ def main():
     wall = kernel_ms = 0.0
     tasks = branches = nbytes = launches = 0
     for i in range(args.steps):
-        w, st, kms = run_unit(eng, root, ncov, learned)
+        w, st, kms, nl = run_unit(eng, root, ncov, learned)
         counts = (st.tasks_expanded, st.branches, st.raw_cliques, st.truncated_children)
         if ref is None:
             ref = counts
@@ -228,7 +232,7 @@ def main():
         tasks += st.tasks_expanded
         branches += st.branches
         nbytes += st.bytes_scanned
-        launches += st.launches
+        launches += nl
         print(f"bench: step {i + 1}/{args.steps}: {w:.2f}s", file=sys.stderr, flush=True)
     import tempfile
     with tempfile.TemporaryDirectory() as tmp:
@@ -247,7 +251,7 @@ def main():
     sched = {}
     if not args.no_schedules:
         for name, sc in (("fifo", dq.SCHED_FIFO), ("size", dq.SCHED_SIZE)):
-            w, st, kms = run_unit(eng, root, ncov, sc)
+            w, st, kms, _ = run_unit(eng, root, ncov, sc)
             counts = (st.tasks_expanded, st.branches, st.raw_cliques, st.truncated_children)
             if counts != ref:
                 raise RuntimeError(f"{name} disagrees with the learned form on the unit: {counts} vs {ref}")
@@ -294,7 +298,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": PMC_TRAFFIC_PER_STEP,
-            "kernel": "expand_kernel<true> (HBM-bitmap variant) and its level-0 split launches, all timed steps",
+            "traffic_source": PMC_TRAFFIC_SOURCE,
+            "kernel": ("expand_kernel, HBM-bitmap variant: per step one level-0 head launch, the level-0 window, "
+                       "the split passes (1024-thread workgroups) and the pieces; achieved = algorithmic bytes per "
+                       "step / their HIP-event time per step"),
             "algorithmic_bytes_per_step": nbytes / max(args.steps, 1),
         },
     }
