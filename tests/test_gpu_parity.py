@@ -122,6 +122,26 @@ def test_split_everything_same_result(dqc, manifest, tmp_path):
     assert not bad, bad
 
 
+def test_split_everything_on_workgroups(dqc, manifest, tmp_path):
+    """Every task with 2 or more branches cut into pieces of 3, on 256-thread workgroups
+    (no wavefront kernel): the split passes of each batch run first, on 1024-thread
+    workgroups, then the pieces. Same bytes and counts."""
+    cfg = dqc.Config()
+    cfg.host_addfail_check = 1
+    cfg.split_branches = 2
+    cfg.split_chunk = 3
+    cfg.wave_tasks = -1
+    bad = []
+    for c in manifest["cases"]:
+        cnt, st, mx = run_case(dqc, c, tmp_path, config=cfg)
+        got = open(mx, "rb").read() if os.path.exists(mx) else b""
+        exp = open(os.path.join(GOLDEN, c["expected"]), "rb").read() if c["expected"] else b""
+        if (got != exp or cnt != c["maximal"] or st.tasks_expanded != c["tasks"] or st.branches != c["branches"]
+                or st.raw_cliques != c["raw_cliques"]):
+            bad.append((c["name"], cnt, c["maximal"], st.tasks_expanded, c["tasks"], st.branches, c["branches"]))
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("summary_shift", [0, 4])
 def test_hbm_bitmaps_same_result(dqc, manifest, tmp_path, monkeypatch, summary_shift):
     """The kernel variant for condensed graphs above 16384 vertices keeps its id bitmaps in

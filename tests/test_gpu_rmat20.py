@@ -47,21 +47,40 @@ def oracle_w4(w4):
     t.close()
 
 
-def test_level0_window_matches_oracle(dqc, w4, oracle_w4):
+@pytest.fixture(scope="module")
+def eng4(w4):
+    """One handle prepared once (45 s of host preprocessing) for every test of this file;
+    each test re-aims its bounded unit (dqc_set_unit) and starts from the level-0 task or
+    from the tasks it pushes. SIZE schedule, tasks of 64+ steps split into pieces of 8+."""
+    import cudqc_amd as dq
+    cfg = dq.Config()
+    cfg.host_addfail_check = 1
+    cfg.schedule = dq.SCHED_SIZE
+    cfg.split_branches, cfg.split_chunk = 64, 8
+    eng = dq.Engine(cfg)
+    print("engine: preparing the R-MAT 2^20 graph", flush=True)
+    k = eng.prepare(w4, GO, GI, MN)
+    root, _, _, ncov = eng.task_meta()[0]
+    yield eng, k, root, ncov
+    eng.close()
+
+
+def run_window(eng, root, ncov, first, count):
+    eng.set_unit(first, count, 0)
+    eng.reset()
+    eng.push_root(root, ncov)
+    eng.step(1)
+    return eng.tasks(), eng.cliques(), eng.stats()
+
+
+def test_level0_window_matches_oracle(dqc, eng4, oracle_w4):
     """The first level-0 steps, branch-parallel on the GPU, equal the oracle's sequential
     loop over the same steps (the bounded unit of work root_branches defines)."""
     _, root, ncov, okids, ocl, obr, _ = oracle_w4
-    cfg = dqc.Config()
-    cfg.host_addfail_check = 1
-    cfg.root_branches = STEPS
-    with dqc.Engine(cfg) as eng:
-        k = eng.prepare(w4, GO, GI, MN)
-        assert k == len(root) == 132279
-        assert eng.task_meta()[0][3] == ncov
-        eng.step(1)
-        tasks = eng.tasks()
-        cl = eng.cliques()
-        st = eng.stats()
+    eng, k, groot, gncov = eng4
+    assert k == len(root) == 132279
+    assert gncov == ncov
+    tasks, cl, st = run_window(eng, groot, gncov, 0, STEPS)
     print(f"level 0: {len(okids)} children, {obr} branches, {len(ocl)} cliques", flush=True)
     assert okids and st.branches == obr
     assert sorted((tuple(r), nm) for r, nm, _ in tasks) == sorted((tuple(r), nm) for r, nm in okids)
@@ -70,20 +89,13 @@ def test_level0_window_matches_oracle(dqc, w4, oracle_w4):
     assert sorted(tuple(sorted(c)) for c in cl) == ocl
 
 
-def test_level0_window_in_dense_part_matches_oracle(dqc, w4, oracle_w4):
+def test_level0_window_in_dense_part_matches_oracle(dqc, eng4, oracle_w4):
     """Level-0 steps [90112, 90124) (the bounded unit of work root_first / root_branches
     define; the steps before run only until the first child): the same children, cliques
     and branch count as the oracle's loop over that window, all children device tasks."""
     _, root, ncov, _, _, _, (wkids, wcl, wbr) = oracle_w4
-    cfg = dqc.Config()
-    cfg.host_addfail_check = 1
-    cfg.root_first, cfg.root_branches = WINDOW
-    with dqc.Engine(cfg) as eng:
-        eng.prepare(w4, GO, GI, MN)
-        eng.step(1)
-        tasks = eng.tasks()
-        cl = eng.cliques()
-        st = eng.stats()
+    eng, _, groot, gncov = eng4
+    tasks, cl, st = run_window(eng, groot, gncov, *WINDOW)
     print(f"window: {len(wkids)} children, {wbr} branches, {len(wcl)} cliques", flush=True)
     assert len(wkids) >= 8 and st.branches == wbr
     assert sorted((tuple(r), nm) for r, nm, _ in tasks) == sorted((tuple(r), nm) for r, nm in wkids)
@@ -91,35 +103,33 @@ def test_level0_window_in_dense_part_matches_oracle(dqc, w4, oracle_w4):
     assert sorted(tuple(sorted(c)) for c in cl) == wcl
 
 
-def test_depth1_tasks_match_oracle(dqc, w4, oracle_w4):
+def test_depth1_tasks_match_oracle(dqc, eng4, oracle_w4):
     """The depth-1 tasks of the dense window, each split into pieces on the GPU (so the
-    whole chip runs it) and compared with the oracle's unsplit expansion (the oracle runs
-    them side by side on the host cores meanwhile)."""
+    whole chip runs it: the split pass on a 1024-thread workgroup, then its pieces) and
+    compared with the oracle's unsplit expansion (the oracle runs them side by side on
+    the host cores meanwhile)."""
     ot, _, _, _, _, _, (wkids, _, _) = oracle_w4
     sample = list(range(len(wkids)))[:10]
     assert len(sample) >= 8, len(wkids)
     print("sample sizes", [len(wkids[i][0]) for i in sample], flush=True)
     with ThreadPoolExecutor(max_workers=len(sample)) as pool:
         futs = {i: pool.submit(ot.expand_digest, wkids[i][0], wkids[i][1], 0, orc.Tasks.DEVICE) for i in sample}
-        cfg = dqc.Config()
-        cfg.host_addfail_check = 1
-        cfg.schedule = dqc.SCHED_SIZE
-        cfg.split_branches, cfg.split_chunk = 64, 8
+        eng, _, _, _ = eng4
+        eng.set_unit(0, 0, 0)
+        eng.set_task_log(1 << 16)
         got = {}
-        with dqc.Engine(cfg) as eng:
-            eng.prepare(w4, GO, GI, MN)
-            for i in sample:
-                recs, nm = wkids[i]
-                eng.reset()
-                eng.push_task(recs, nm, 0, orc.Tasks.DEVICE)
-                eng.step(1)  # the split pass: pieces only
-                assert eng.stats().tasks_expanded == 1 and eng.stats().branches == 0, i
-                eng.step(1)  # every piece (one batch holds them all)
-                st = eng.stats()
-                assert st.stack_pushbacks == 0 and st.time_slices == 0, i
-                got[i] = (sorted((h, m) for h, m, _ in eng.task_digests()),
-                          sorted(tuple(sorted(c)) for c in eng.cliques()), st.branches, st.tasks_expanded)
-                print(f"task {i}: size {len(recs)}, {st.branches} branches, {len(got[i][0])} children", flush=True)
+        for i in sample:
+            recs, nm = wkids[i]
+            eng.reset()
+            eng.push_task(recs, nm, 0, orc.Tasks.DEVICE)
+            eng.run()
+            st = eng.stats()
+            log, _ = eng.task_log()
+            assert sum(1 for r in log if r.flags & 1) >= 2, i  # it ran as pieces
+            assert st.stack_pushbacks == 0 and st.time_slices == 0, i
+            got[i] = (sorted((h, m) for h, m, _ in eng.task_digests()),
+                      sorted(tuple(sorted(c)) for c in eng.cliques()), st.branches, st.tasks_expanded)
+            print(f"task {i}: size {len(recs)}, {st.branches} branches, {len(got[i][0])} children", flush=True)
         for i in sample:
             kids, cl, br = futs[i].result()
             assert got[i][3] == 1, i
