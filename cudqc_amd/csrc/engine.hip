@@ -304,6 +304,9 @@ struct W {
         adjc = vbase + 9 * k - gb;
     }
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
+    // lvl2adj decrements of removals are applied when a value is next read: the removals
+    // of positions [ptot, l2_tot) are pending (ptot: the parent's live count)
+    int ptot, l2_tot;
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
     uint32_t nbranch;             // branches run for the current task / piece
     uint32_t launch_steps;        // stack checks this workgroup passed in this launch
@@ -867,6 +870,7 @@ struct W {
             if (tid == 0) atomicAdd(&P->g->truncated, 1ull);
             return -1;
         }
+        flush_l2(ptot);  // the child's records carry the parent's lvl2adj
         // key field widths from this child's maxima
         uint32_t mmx = 0, mcx = 0, l2x = 0;
         uint32_t omin = UINT_MAX, omax = 0, imin = UINT_MAX, imax = 0;  // degree features
@@ -1049,6 +1053,7 @@ struct W {
     // first child (the host level-1 task) is already written.
     __device__ __forceinline__ void write_piece(const TaskDesc& src, int tot, int nm, int mode, bool remove_first, int iters,
                                                 bool deferred, int step0, bool l1_done) {
+        flush_l2(tot);
         unsigned long long a = 0;
         if (tid == 0) a = reserve(deferred, (unsigned long long)tot);
         a = bcast0(a);
@@ -1104,24 +1109,38 @@ struct W {
                 if (pom[w] + oc < mo) { if (memb) fail = 1; else if (w == last) skip = 1; }
             }
         });
-        // two-hop list: the live non-members in it (collected into gat) lose one lvl2adj
-        wg_collect(P->tn, P->to[p], P->to[p + 1], plive, pmem, -1, gat, 9);
+        // the two-hop list's lvl2adj decrements are left pending (flush_l2); its bytes are
+        // counted here, as the reference's removal scans it
         if (tid == 0)
             bytes += 4ull * ((P->oo[p + 1] - P->oo[p]) + (P->io[p + 1] - P->io[p]) + (P->to[p + 1] - P->to[p])) + 48;
-        const int bad = __syncthreads_or(fail) ? 1 : __syncthreads_or(skip) ? 2 : 0;  // barriers: gat complete
-        const int nh = S->cnt[9];
-        for (int i = tid; i < nh; i += NT) {
-            const int w = gat[i];
-            pl2[w] -= 1u;
+        if (__syncthreads_or(fail)) return 0;
+        return __syncthreads_or(skip) ? 2 : 1;
+    }
+
+    // Applies the pending lvl2adj decrements of the removals of positions [tot, l2_tot)
+    // (reference src/device_kernels.cu:513-524: a removal decrements lvl2adj of the live
+    // non-members in the removed vertex's two-hop list). Only the values of vertices still
+    // live are read again, and a vertex live now was live at each of those removals, so
+    // applying them together gives the values of applying them one removal at a time.
+    // Called before every read of lvl2adj: a task whose branches write no child and whose
+    // lookahead fails on degrees never needs them.
+    __device__ __forceinline__ void flush_l2(int tot) {
+        for (int i = tot; i < l2_tot; ++i) {
+            const int p = pids[i];
+            wg_collect(P->tn, P->to[p], P->to[p + 1], plive, pmem, -1, gat, 9);
+            __syncthreads();
+            const int nh = S->cnt[9];
+            for (int j = tid; j < nh; j += NT) pl2[gat[j]] -= 1u;
+            __syncthreads();
+            if (tid == 0) S->cnt[9] = 0;
+            __syncthreads();
         }
-        __syncthreads();
-        if (tid == 0) S->cnt[9] = 0;
-        if (bad == 1) return 0;
-        return bad == 2 ? 2 : 1;
+        l2_tot = tot;
     }
 
     // reference src/host_functions.cu:1333-1383 (host: lvl2adj >= tot - 1) and
-    // src/device_kernels.cu:330-397 (device: lvl2adj >= num_cand - 1)
+    // src/device_kernels.cu:330-397 (device: lvl2adj >= num_cand - 1). The degree tests
+    // come first: lvl2adj is brought up to date only when all of them pass.
     __device__ __forceinline__ bool lookahead(int tot, int nm, int nc, int mode) {
         const uint32_t mo = (uint32_t)gmd(tot, P->mdo), mi = (uint32_t)gmd(tot, P->mdi);
         const uint32_t l2min = mode == MODE_DEVICE ? (uint32_t)(nc - 1) : (uint32_t)(tot - 1);
@@ -1129,8 +1148,14 @@ struct W {
             int i = base + tid, bad = 0;
             if (i < tot) {
                 int x = pids[i];
-                bad = (pom[x] + poc[x] < mo) || (pim[x] + pic[x] < mi) || (i >= nm && pl2[x] < l2min);
+                bad = (pom[x] + poc[x] < mo) || (pim[x] + pic[x] < mi);
             }
+            if (__syncthreads_or(bad)) return false;
+        }
+        flush_l2(tot);
+        for (int base = nm; base < tot; base += NT) {
+            const int i = base + tid;
+            const int bad = i < tot && pl2[pids[i]] < l2min;
             if (__syncthreads_or(bad)) return false;
         }
         return true;
@@ -1142,6 +1167,7 @@ struct W {
     // others are dropped by diameter pruning before any of their values is read.
     // Returns true if a child was written.
     __device__ __forceinline__ bool branch(int tot, int nm, int mode, bool first_child_host) {
+        ptot = tot;
         const int v = pids[tot - 1];
         const int cm = nm + 1;
         if (cm + 1 >= HMAX) {  // member histograms sized for fewer members
@@ -1212,6 +1238,7 @@ struct W {
     // vertex (reference src/host_functions.cu:1386-1500, 1503-1559).
     __device__ __forceinline__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
         const int v = k - s - 1, tot = k - s;
+        ptot = l2_tot = tot;  // the values below are computed as of step s
         begin_child(1);
         dirty_m = 1;
         const unsigned long long *oo = P->oo, *io = P->io;
@@ -1418,6 +1445,7 @@ struct W {
             if (i < nm) bs(pmem, x);
         }
         bytes += 24ull * T / NT + 1;
+        ptot = l2_tot = T;  // no lvl2adj decrement pending
         if (tid == 0 && !piece) atomicAdd(&P->g->tasks, 1ull);
         __syncthreads();
         int tot = T, nc = T - nm;
