@@ -304,9 +304,36 @@ struct W {
         adjc = vbase + 9 * k - gb;
     }
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
+    bool crit_marks;       // critical() ran: members and ccrit bits among lst too
     // lvl2adj decrements of removals are applied when a value is next read: the removals
     // of positions [ptot, l2_tot) are pending (ptot: the parent's live count)
     int ptot, l2_tot;
+    // Branch setup leaves the child degrees of the L(v) candidates implicit: the parent's,
+    // with v's arcs moved from candidate to member (cL marks v's live in-neighbours, ccrit
+    // its live out-neighbours). They are written only for a branch that passes its first
+    // bound test (materialize). lazy_v = v while they are implicit, else -1.
+    int lazy_v;
+    __device__ __forceinline__ uint32_t in_v(int w) const { return bt(cL, w) ? 1u : 0u; }    // arc w -> v
+    __device__ __forceinline__ uint32_t out_v(int w) const { return bt(ccrit, w) ? 1u : 0u; } // arc v -> w
+    // writes the child degrees of the candidates R[0..n) (implicit until now)
+    __device__ __forceinline__ void materialize(const int32_t* R, int n) {
+        for (int i = tid; i < n; i += NT) {
+            const int w = R[i];
+            const uint32_t a = in_v(w), b = out_v(w);
+            com[w] = pom[w] + a; coc[w] = poc[w] - a; cim[w] = pim[w] + b; cic[w] = pic[w] - b;
+        }
+        __syncthreads();
+    }
+    // clears the marks of v's neighbours (cL, ccrit) once the implicit degrees are settled
+    __device__ __forceinline__ void clear_v_marks(int v) {
+        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], ccrit, nullptr, [&](int w, uint32_t c, uint32_t) {
+            if (c) bcl(ccrit, w);
+        });
+        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cL, nullptr, [&](int w, uint32_t c, uint32_t) {
+            if (c) bcl(cL, w);
+        });
+        __syncthreads();
+    }
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
     uint32_t nbranch;             // branches run for the current task / piece
     uint32_t launch_steps;        // stack checks this workgroup passed in this launch
@@ -803,6 +830,7 @@ struct W {
         int rc = *nr;
         if (!lu_bounds(cm, rc, ub, lb, meo, mei)) return false;
         if (!members_ext(cm, ub, lb, meo, mei)) return false;
+        if (lazy_v >= 0) materialize(*R, rc);
         int nd = 0;
         int32_t* cur = *R;
         int k = filter(cur, rc, spare, dropped, &nd, cm, ub, lb, meo, mei);
@@ -1183,29 +1211,28 @@ struct W {
             com[x] = pom[x]; coc[x] = poc[x]; cim[x] = pim[x]; cic[x] = pic[x];
             bs(cmem, x);
         }
-        // candidates within two hops of v (the others fail diameter pruning): collected
-        // into lst, then their degrees copied
+        // candidates within two hops of v (the others fail diameter pruning), collected into
+        // lst; their child degrees stay implicit (lazy_v)
         wg_collect(P->tn, P->to[v], P->to[v + 1], plive, pmem, v, lst, 3);
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
         __syncthreads();
         const int nl = S->cnt[3];
-        for (int i = tid; i < nl; i += NT) {
-            const int w = lst[i];
-            com[w] = pom[w]; coc[w] = poc[w]; cim[w] = pim[w]; cic[w] = pic[w];
-            bs(cL, w);
-        }
-        __syncthreads();
         dirty_l = nl;
-        // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members
-        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], cmem, cL, [&](int w, uint32_t m, uint32_t l) {
-            if (m | l) { cim[w] += 1u; cic[w] -= 1u; }
+        // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members.
+        // Members' degrees are updated; v's live candidate neighbours are marked.
+        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], cmem, plive, [&](int w, uint32_t m, uint32_t l) {
+            if (m) { cim[w] += 1u; cic[w] -= 1u; }
+            else if (l) bs(ccrit, w);
         });
-        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cmem, cL, [&](int w, uint32_t m, uint32_t l) {
-            if (m | l) { com[w] += 1u; coc[w] -= 1u; }
+        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cmem, plive, [&](int w, uint32_t m, uint32_t l) {
+            if (m) { com[w] += 1u; coc[w] -= 1u; }
+            else if (l) bs(cL, w);
         });
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
+        lazy_v = v;
         const long long t = branch_tail(nm, tot, nl, mode, first_child_host ? MODE_HOST_L1 : MODE_DEVICE, -1);
+        lazy_v = -1;
         end_child();
         return t >= 0 || t == -2;  // -2: a child outside the bounded unit (still the first child)
     }
@@ -1220,14 +1247,21 @@ struct W {
     // across launches when they live in HBM.
     __device__ __forceinline__ void end_child() {
         __syncthreads();
+        // (cL and v's ccrit marks are cleared when the branch's degree test ends; members
+        // other than those in mem[0..dirty_m) and ccrit bits come from critical())
         for (int i = tid; i < dirty_m + dirty_l; i += NT) {
-            const int x = i < dirty_m ? mem[i] : lst[i - dirty_m];
-            const int wd = (x - gb) >> 5;
-            cmem[wd] = 0; ccand[wd] = 0; cL[wd] = 0; ccrit[wd] = 0;
+            if (i < dirty_m) {
+                cmem[(mem[i] - gb) >> 5] = 0;
+            } else {
+                const int wd = (lst[i - dirty_m] - gb) >> 5;
+                ccand[wd] = 0;
+                if (crit_marks) { cmem[wd] = 0; ccrit[wd] = 0; }
+            }
         }
         __syncthreads();
         dirty_m = 0;
         dirty_l = 0;
+        crit_marks = false;
     }
 
     // Branch s of the level-0 task, built without replaying the s earlier removals.
@@ -1259,7 +1293,6 @@ struct W {
             cic[u] = cnt_lt(P->in, io[u], io[u + 1], v);
             const unsigned long long l0 = P->lto[u], l1 = P->lto[u + 1];
             pl2[u] = root[u].lvl2adj - ((uint32_t)(l1 - l0) - cnt_lt(P->ltn, l0, l1, tot));
-            bs(cL, u);
             lst[atomicAdd(&S->cnt[3], 1)] = u;
         }
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
@@ -1282,11 +1315,18 @@ struct W {
             const uint32_t mo2 = (uint32_t)gmd(nm + 2, P->mdo), mi2 = (uint32_t)gmd(nm + 2, P->mdi);
             for (int i = tid; i < nl; i += NT) {
                 int w = lst[i];
-                if (com[w] + coc[w] >= mo2 && cim[w] + cic[w] >= mi2) {
+                uint32_t om, oc, im, ic;
+                if (lazy_v >= 0) {  // the add moves v's arcs from candidate to member
+                    const uint32_t a = in_v(w), b = out_v(w);
+                    om = pom[w] + a; oc = poc[w] - a; im = pim[w] + b; ic = pic[w] - b;
+                } else {
+                    om = com[w]; oc = coc[w]; im = cim[w]; ic = cic[w];
+                }
+                if (om + oc >= mo2 && im + ic >= mi2) {
                     bs(ccand, w);
                     la[atomicAdd(&S->cnt[4], 1)] = w;
-                    atomicAdd(&S->ho[com[w]], 1);
-                    atomicAdd(&S->hi[cim[w]], 1);
+                    atomicAdd(&S->ho[om], 1);
+                    atomicAdd(&S->hi[im], 1);
                 }
             }
         }
@@ -1296,6 +1336,10 @@ struct W {
         if (tid == 0 && !discard_child) atomicAdd(&P->g->branches, 1ull);  // window branches only
         tick(3);
         bool ok = degree_prune(cm, &R, &nr, tot - 1 - nm, ub, lbd, meo, mei, lb, lc);
+        if (lazy_v >= 0) {  // (degree_prune materialized the degrees if it got past its first test)
+            clear_v_marks(lazy_v);
+            lazy_v = -1;
+        }
         tick(4);
         if (!ok) {
             // reference src/device_kernels.cu:131-132 (always) / src/host_functions.cu:1081
@@ -1323,6 +1367,7 @@ struct W {
     // cm / R / nr are updated in place.
     // On entry R is la or lb (degree_prune alternates those two and drops into lc).
     __device__ __forceinline__ int critical(int& cm, int32_t*& R, int& nr, int& ub, int& lbd, int& meo, int& mei) {
+        crit_marks = true;
         const uint32_t tho = (uint32_t)P->mdo[cm + lbd], thi = (uint32_t)P->mdi[cm + lbd];
         for (int a = 0; a < cm; ++a) {
             int x = mem[a];
@@ -1578,6 +1623,8 @@ void expand_kernel(KParams P) {
     w.plive = bm; w.pmem = bm + bw; w.cmem = bm + 2 * bw; w.ccand = bm + 3 * bw; w.cL = bm + 4 * bw; w.ccrit = bm + 5 * bw;
     w.dirty_m = 0;
     w.dirty_l = 0;
+    w.crit_marks = false;
+    w.lazy_v = -1;
     w.launch_steps = 0;
     w.t_start = __builtin_amdgcn_s_memrealtime();
     w.discard_child = false;
