@@ -216,6 +216,8 @@ struct Shared {
     int32_t lu[12];
     unsigned long long bcast;
     unsigned long long acc_bytes;
+    unsigned long long clip[4];  // list ranges narrowed by clip_live (two pairs, alternating)
+    int32_t idr[2];              // min / max live id of the task being loaded
 };
 
 
@@ -229,6 +231,26 @@ __device__ __forceinline__ uint32_t cnt_lt(const int32_t* a, unsigned long long 
         if (a[m] < x) l = m + 1; else h = m;
     }
     return (uint32_t)(l - lo);
+}
+// First index i of the sorted list a[b, e) with a[i] >= x (e if none), found by the 64
+// lanes of one wavefront together: each round every lane tests one of 64 evenly spaced
+// entries, which narrows the range 64-fold, so a list of 2^18 entries takes three
+// dependent loads. Every lane of the wavefront calls it with the same arguments.
+__device__ __forceinline__ unsigned long long wave_lower_bound(const int32_t* a, unsigned long long b,
+                                                               unsigned long long e, int x, int lane) {
+    while (true) {  // the answer lies in [b, e]
+        const unsigned long long n = e - b;
+        if (n == 0) return b;
+        const unsigned long long step = (n + 63) / 64;
+        const unsigned long long i = b + (unsigned long long)lane * step;
+        const bool ge = i >= e || a[i] >= x;
+        const unsigned long long m = __ballot(ge);
+        const int f = m ? __builtin_ctzll(m) : 64;  // first sampled entry >= x
+        if (step == 1 || f == 0) return b + (unsigned long long)f;
+        const unsigned long long nb = b + (unsigned long long)(f - 1) * step + 1;
+        e = f == 64 ? e : min(e, b + (unsigned long long)f * step);
+        b = nb;
+    }
 }
 __device__ __forceinline__ bool contains(const int32_t* a, unsigned long long lo, unsigned long long hi, int x) {
     unsigned long long l = lo + cnt_lt(a, lo, hi, x);
@@ -275,7 +297,35 @@ struct W {
     // HBM read: the common case, since few neighbours of a vertex are in the task.
     uint32_t* lsum;  // null in the LDS-bitmap variant
     int gsh;
+    // Every id live in the current task lies in [id_lo, id_hi) (the range of its ids when
+    // it was loaded): the entries of a sorted list outside it are in no bitmap, so long
+    // list scans skip them (clip_live).
+    int id_lo, id_hi;
+    int clip_pair;
+    // v's out- and in-lists narrowed to [id_lo, id_hi) (branch setup)
+    unsigned long long vob, voe, vib, vie;
     const uint32_t* zw;  // zero_words
+    // Narrows the sorted list range [b, e) to its entries in [id_lo, id_hi); a list shorter
+    // than 8 entries per thread is left whole. All threads call it with the same range.
+    __device__ __forceinline__ void clip_live(const int32_t* a, unsigned long long& b, unsigned long long& e) {
+        if (e - b < 8ull * NT || (id_lo == INT_MIN && id_hi == INT_MAX)) return;
+        const int wave = tid >> 6, lane = tid & 63;
+        unsigned long long* r = S->clip + clip_pair;
+        if (wave == 0) {
+            const unsigned long long x = wave_lower_bound(a, b, e, id_lo, lane);
+            if (lane == 0) r[0] = x;
+        }
+        if (wave == (NT > 64 ? 1 : 0)) {
+            const unsigned long long x = wave_lower_bound(a, b, e, id_hi, lane);
+            if (lane == 0) r[1] = x;
+        }
+        __syncthreads();
+        // (the other pair is written next: a thread that writes it has passed this barrier
+        // again only after every thread read this pair)
+        b = r[0];
+        e = r[1] > b ? r[1] : b;
+        clip_pair ^= 2;
+    }
     __device__ __forceinline__ bool maybe_live(int id) const {
         const int g = (id - gb) >> gsh;
         return (lsum[g >> 5] >> (g & 31)) & 1u;
@@ -326,10 +376,10 @@ struct W {
     }
     // clears the marks of v's neighbours (cL, ccrit) once the implicit degrees are settled
     __device__ __forceinline__ void clear_v_marks(int v) {
-        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], ccrit, nullptr, [&](int w, uint32_t c, uint32_t) {
+        wg_scan_bits(P->on, vob, voe, ccrit, nullptr, [&](int w, uint32_t c, uint32_t) {
             if (c) bcl(ccrit, w);
         });
-        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cL, nullptr, [&](int w, uint32_t c, uint32_t) {
+        wg_scan_bits(P->in, vib, vie, cL, nullptr, [&](int w, uint32_t c, uint32_t) {
             if (c) bcl(cL, w);
         });
         __syncthreads();
@@ -1123,14 +1173,17 @@ struct W {
         // member falls short; here the in- and two-hop decrements may also have run, which
         // is unobservable because a failure ends the task.
         int fail = 0, skip = 0;
-        wg_scan_bits(P->on, P->oo[p], P->oo[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+        unsigned long long ob = P->oo[p], oe = P->oo[p + 1], ib = P->io[p], ie = P->io[p + 1];
+        clip_live(P->on, ob, oe);
+        clip_live(P->in, ib, ie);
+        wg_scan_bits(P->on, ob, oe, plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
             if (live) {
                 uint32_t ic = pic[w] - 1u;
                 pic[w] = ic;
                 if (pim[w] + ic < mi) { if (memb) fail = 1; else if (w == last) skip = 1; }
             }
         });
-        wg_scan_bits(P->in, P->io[p], P->io[p + 1], plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
+        wg_scan_bits(P->in, ib, ie, plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
             if (live) {
                 uint32_t oc = poc[w] - 1u;
                 poc[w] = oc;
@@ -1155,7 +1208,9 @@ struct W {
     __device__ __forceinline__ void flush_l2(int tot) {
         for (int i = tot; i < l2_tot; ++i) {
             const int p = pids[i];
-            wg_collect(P->tn, P->to[p], P->to[p + 1], plive, pmem, -1, gat, 9);
+            unsigned long long b = P->to[p], e = P->to[p + 1];
+            clip_live(P->tn, b, e);
+            wg_collect(P->tn, b, e, plive, pmem, -1, gat, 9);
             __syncthreads();
             const int nh = S->cnt[9];
             for (int j = tid; j < nh; j += NT) pl2[gat[j]] -= 1u;
@@ -1213,18 +1268,25 @@ struct W {
         }
         // candidates within two hops of v (the others fail diameter pruning), collected into
         // lst; their child degrees stay implicit (lazy_v)
-        wg_collect(P->tn, P->to[v], P->to[v + 1], plive, pmem, v, lst, 3);
+        {
+            unsigned long long b = P->to[v], e = P->to[v + 1];
+            clip_live(P->tn, b, e);
+            wg_collect(P->tn, b, e, plive, pmem, v, lst, 3);
+        }
         if (tid == 0) bytes += 4ull * (P->to[v + 1] - P->to[v]) + 16;
+        vob = P->oo[v]; voe = P->oo[v + 1]; vib = P->io[v]; vie = P->io[v + 1];
+        clip_live(P->on, vob, voe);
+        clip_live(P->in, vib, vie);
         __syncthreads();
         const int nl = S->cnt[3];
         dirty_l = nl;
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members.
         // Members' degrees are updated; v's live candidate neighbours are marked.
-        wg_scan_bits(P->on, P->oo[v], P->oo[v + 1], cmem, plive, [&](int w, uint32_t m, uint32_t l) {
+        wg_scan_bits(P->on, vob, voe, cmem, plive, [&](int w, uint32_t m, uint32_t l) {
             if (m) { cim[w] += 1u; cic[w] -= 1u; }
             else if (l) bs(ccrit, w);
         });
-        wg_scan_bits(P->in, P->io[v], P->io[v + 1], cmem, plive, [&](int w, uint32_t m, uint32_t l) {
+        wg_scan_bits(P->in, vib, vie, cmem, plive, [&](int w, uint32_t m, uint32_t l) {
             if (m) { com[w] += 1u; coc[w] -= 1u; }
             else if (l) bs(cL, w);
         });
@@ -1273,6 +1335,7 @@ struct W {
     __device__ __forceinline__ long long branch_root(int s, int k, const dqc::TaskVertex* root) {
         const int v = k - s - 1, tot = k - s;
         ptot = l2_tot = tot;  // the values below are computed as of step s
+        id_lo = INT_MIN; id_hi = INT_MAX;  // (the live set is [0, tot): scans test ids themselves)
         begin_child(1);
         dirty_m = 1;
         const unsigned long long *oo = P->oo, *io = P->io;
@@ -1284,7 +1347,17 @@ struct W {
             pl2[v] = root[v].lvl2adj - ((uint32_t)(P->lto[v + 1] - P->lto[v]) - cnt_lt(P->ltn, P->lto[v], P->lto[v + 1], tot));
             bs(cmem, v);
         }
-        for (unsigned long long e = P->to[v] + tid; e < P->to[v + 1]; e += NT) {
+        unsigned long long tb = P->to[v], te = P->to[v + 1];
+        if (te - tb >= 8ull * NT) {  // only the entries below v are live
+            if (tid < 64) {
+                const unsigned long long x = wave_lower_bound(P->tn, tb, te, v, tid);
+                if (tid == 0) S->clip[clip_pair] = x;
+            }
+            __syncthreads();
+            te = S->clip[clip_pair];
+            clip_pair ^= 2;
+        }
+        for (unsigned long long e = tb + tid; e < te; e += NT) {
             int u = P->tn[e];
             if (u >= v) continue;
             com[u] = contains(P->in, io[v], io[v + 1], u) ? 1u : 0u;   // u -> v
@@ -1480,9 +1553,13 @@ struct W {
         cur_uid = d.uid;
         cur_depth = d.depth;
         nbranch = 0;
+        if (tid == 0) { S->idr[0] = INT_MAX; S->idr[1] = INT_MIN; }
+        __syncthreads();
+        int lo = INT_MAX, hi = INT_MIN;
         for (int i = tid; i < T; i += NT) {
             dqc::TaskVertex r = rv[i];
             int x = (int)r.id;
+            lo = min(lo, x); hi = max(hi, x);
             pids[i] = x;
             pom[x] = r.out_mem; poc[x] = r.out_can; pim[x] = r.in_mem; pic[x] = r.in_can; pl2[x] = r.lvl2adj;
             bs(plive, x);
@@ -1492,7 +1569,10 @@ struct W {
         bytes += 24ull * T / NT + 1;
         ptot = l2_tot = T;  // no lvl2adj decrement pending
         if (tid == 0 && !piece) atomicAdd(&P->g->tasks, 1ull);
+        if (lo <= hi) { atomicMin(&S->idr[0], lo); atomicMax(&S->idr[1], hi); }
         __syncthreads();
+        id_lo = S->idr[0];
+        id_hi = S->idr[1] + 1;
         int tot = T, nc = T - nm;
         int expansions = piece ? min(d.iters, nc) : nc;
         // a level-0 piece resumes its task's loop at step step0 (window bounds count from there)
@@ -1625,6 +1705,8 @@ void expand_kernel(KParams P) {
     w.dirty_l = 0;
     w.crit_marks = false;
     w.lazy_v = -1;
+    w.id_lo = INT_MIN; w.id_hi = INT_MAX;
+    w.clip_pair = 0;
     w.launch_steps = 0;
     w.t_start = __builtin_amdgcn_s_memrealtime();
     w.discard_child = false;
