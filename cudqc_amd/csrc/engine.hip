@@ -355,6 +355,7 @@ struct W {
     }
     int dirty_m, dirty_l;  // child bitmaps: members mem[0..dirty_m) and lst[0..dirty_l) are set
     bool crit_marks;       // critical() ran: members and ccrit bits among lst too
+    bool cand_marks;       // ccand bits were set among lst
     // lvl2adj decrements of removals are applied when a value is next read: the removals
     // of positions [ptot, l2_tot) are pending (ptot: the parent's live count)
     int ptot, l2_tot;
@@ -365,13 +366,16 @@ struct W {
     int lazy_v;
     __device__ __forceinline__ uint32_t in_v(int w) const { return bt(cL, w) ? 1u : 0u; }    // arc w -> v
     __device__ __forceinline__ uint32_t out_v(int w) const { return bt(ccrit, w) ? 1u : 0u; } // arc v -> w
-    // writes the child degrees of the candidates R[0..n) (implicit until now)
+    // writes the child degrees of the candidates R[0..n) (implicit until now) and sets their
+    // candidate bits (the first bound test reads neither)
     __device__ __forceinline__ void materialize(const int32_t* R, int n) {
         for (int i = tid; i < n; i += NT) {
             const int w = R[i];
             const uint32_t a = in_v(w), b = out_v(w);
             com[w] = pom[w] + a; coc[w] = poc[w] - a; cim[w] = pim[w] + b; cic[w] = pic[w] - b;
+            bs(ccand, w);
         }
+        cand_marks = true;
         __syncthreads();
     }
     // clears the marks of v's neighbours (cL, ccrit) once the implicit degrees are settled
@@ -1311,12 +1315,13 @@ struct W {
         __syncthreads();
         // (cL and v's ccrit marks are cleared when the branch's degree test ends; members
         // other than those in mem[0..dirty_m) and ccrit bits come from critical())
-        for (int i = tid; i < dirty_m + dirty_l; i += NT) {
+        const int nclear = dirty_m + (cand_marks || crit_marks ? dirty_l : 0);
+        for (int i = tid; i < nclear; i += NT) {
             if (i < dirty_m) {
                 cmem[(mem[i] - gb) >> 5] = 0;
             } else {
                 const int wd = (lst[i - dirty_m] - gb) >> 5;
-                ccand[wd] = 0;
+                if (cand_marks) ccand[wd] = 0;
                 if (crit_marks) { cmem[wd] = 0; ccrit[wd] = 0; }
             }
         }
@@ -1324,6 +1329,7 @@ struct W {
         dirty_m = 0;
         dirty_l = 0;
         crit_marks = false;
+        cand_marks = false;
     }
 
     // Branch s of the level-0 task, built without replaying the s earlier removals.
@@ -1386,6 +1392,7 @@ struct W {
         // reference src/device_kernels.cu:830-906 (d_diameter_pruning): basic degree test
         {
             const uint32_t mo2 = (uint32_t)gmd(nm + 2, P->mdo), mi2 = (uint32_t)gmd(nm + 2, P->mdi);
+            if (lazy_v < 0) cand_marks = true;
             for (int i = tid; i < nl; i += NT) {
                 int w = lst[i];
                 uint32_t om, oc, im, ic;
@@ -1396,7 +1403,7 @@ struct W {
                     om = com[w]; oc = coc[w]; im = cim[w]; ic = cic[w];
                 }
                 if (om + oc >= mo2 && im + ic >= mi2) {
-                    bs(ccand, w);
+                    if (lazy_v < 0) bs(ccand, w);  // (else when materialized)
                     la[atomicAdd(&S->cnt[4], 1)] = w;
                     atomicAdd(&S->ho[om], 1);
                     atomicAdd(&S->hi[im], 1);
@@ -1704,6 +1711,7 @@ void expand_kernel(KParams P) {
     w.dirty_m = 0;
     w.dirty_l = 0;
     w.crit_marks = false;
+    w.cand_marks = false;
     w.lazy_v = -1;
     w.id_lo = INT_MIN; w.id_hi = INT_MAX;
     w.clip_pair = 0;
