@@ -1209,20 +1209,54 @@ struct W {
     // applying them together gives the values of applying them one removal at a time.
     // Called before every read of lvl2adj: a task whose branches write no child and whose
     // lookahead fails on degrees never needs them.
+    // The pending two-hop lists are shared out among the wavefronts (each takes the next
+    // list when it is done with one), so the workgroup has the entries of several lists in
+    // flight and no barrier between lists; a vertex in several of them takes atomic
+    // decrements.
     __device__ __forceinline__ void flush_l2(int tot) {
-        for (int i = tot; i < l2_tot; ++i) {
-            const int p = pids[i];
-            unsigned long long b = P->to[p], e = P->to[p + 1];
-            clip_live(P->tn, b, e);
-            wg_collect(P->tn, b, e, plive, pmem, -1, gat, 9);
+        const int np = l2_tot - tot;
+        if (np > 0) {
+            if (tid == 0) S->cnt[10] = 0;
             __syncthreads();
-            const int nh = S->cnt[9];
-            for (int j = tid; j < nh; j += NT) pl2[gat[j]] -= 1u;
-            __syncthreads();
-            if (tid == 0) S->cnt[9] = 0;
+            const int lane = tid & 63;
+            const bool ranged = !(id_lo == INT_MIN && id_hi == INT_MAX);
+            while (true) {
+                int j = 0;
+                if (lane == 0) j = atomicAdd(&S->cnt[10], 1);
+                j = __shfl(j, 0, 64);
+                if (j >= np) break;
+                const int p = pids[tot + j];
+                unsigned long long b = P->to[p], e = P->to[p + 1];
+                if (ranged && e - b >= 512) {  // only ids in [id_lo, id_hi) can be live
+                    b = wave_lower_bound(P->tn, b, e, id_lo, lane);
+                    e = wave_lower_bound(P->tn, b, e, id_hi, lane);
+                }
+                wave_decrement_l2(P->tn + b, (int)(e - b), lane);
+            }
             __syncthreads();
         }
         l2_tot = tot;
+    }
+    // One wavefront: lvl2adj -= 1 for every live non-member in the list a[0, n), with
+    // SCAN_UNR entries per lane and their bitmap words in flight per round.
+    __device__ __forceinline__ void wave_decrement_l2(const int32_t* __restrict__ a, int n, int lane) {
+        if (n <= 0) return;
+        constexpr int U = SCAN_UNR;
+        const int last = n - 1;
+        for (int k = lane; k < n; k += 64 * U) {
+            int v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = a[min(k + u * 64, last)];
+            uint32_t wa[U], wb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                wa[u] = *bword_ptr(plive, v[u]);
+                wb[u] = *bword_ptr(pmem, v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k + u * 64 < n && bbit(wa[u], v[u]) && !bbit(wb[u], v[u])) atomicSub(&pl2[v[u]], 1u);
+        }
     }
 
     // reference src/host_functions.cu:1333-1383 (host: lvl2adj >= tot - 1) and
@@ -1669,7 +1703,8 @@ struct W {
 // The wavefront kernel's LDS (8 KB per wavefront) allows 5 waves per SIMD, and its
 // register budget is set for that many.
 template <bool BIG, int TPB = NT, int LKW = KW>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(TPB == WAVE_T ? 5 : DQC_WAVES_PER_EU)))
+__global__ __launch_bounds__(TPB)
+__attribute__((amdgpu_waves_per_eu(TPB == WAVE_T ? 5 : TPB == SPLIT_T ? 4 : DQC_WAVES_PER_EU)))
 void expand_kernel(KParams P) {
     if (P.in_ntask == ~0ull && blockIdx.x >= P.q->wg_limit) return;  // no stack room for them
     __shared__ Shared S;
