@@ -378,14 +378,17 @@ struct W {
         cand_marks = true;
         __syncthreads();
     }
-    // clears the marks of v's neighbours (cL, ccrit) once the implicit degrees are settled
-    __device__ __forceinline__ void clear_v_marks(int v) {
-        wg_scan_bits(P->on, vob, voe, ccrit, nullptr, [&](int w, uint32_t c, uint32_t) {
-            if (c) bcl(ccrit, w);
-        });
-        wg_scan_bits(P->in, vib, vie, cL, nullptr, [&](int w, uint32_t c, uint32_t) {
-            if (c) bcl(cL, w);
-        });
+    // clears the marks of v's neighbours once the implicit degrees are settled: until
+    // critical() runs, every bit of cL and ccrit is such a mark, so the words of v's
+    // neighbours are zeroed whole (plain stores, no read)
+    __device__ __forceinline__ void clear_v_marks() {
+        const long long n1 = (long long)(voe - vob), n = n1 + (long long)(vie - vib);
+        for (long long k = tid; k < n; k += NT) {
+            const bool in_list = k >= n1;
+            const int w = in_list ? P->in[vib + (k - n1)] : P->on[vob + k];
+            if (lsum && !maybe_live(w)) continue;  // (only live ids were marked)
+            (in_list ? cL : ccrit)[(w - gb) >> 5] = 0u;
+        }
         __syncthreads();
     }
     uint32_t cur_uid, cur_depth;  // the task being expanded (parent of the children written)
@@ -488,6 +491,33 @@ struct W {
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = v2[u];
             k = k2;
+        }
+    }
+
+    // As wg_scan_bits over the two lists a1[b1, e1) and a2[b2, e2) as one range (one round of
+    // loads for both when they are short): f(id, bit in bmA, bit in bmB, entry of the second).
+    template <class F>
+    __device__ __forceinline__ void wg_scan_bits2(const int32_t* __restrict__ a1, unsigned long long b1,
+                                                  unsigned long long e1, const int32_t* __restrict__ a2,
+                                                  unsigned long long b2, unsigned long long e2,
+                                                  const uint32_t* bmA, const uint32_t* bmB, F&& f) const {
+        const long long n1 = (long long)(e1 - b1), n = n1 + (long long)(e2 - b2);
+        for (long long k = tid; k < n; k += (long long)SCAN_UNR * NT) {
+            int v[SCAN_UNR];
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u) {
+                const long long j = k + (long long)u * NT;
+                v[u] = j < n1 ? a1[b1 + j] : (j < n ? a2[b2 + (j - n1)] : -1);
+            }
+            uint32_t wa[SCAN_UNR], wb[SCAN_UNR];
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u) {
+                wa[u] = v[u] >= 0 ? bword(bmA, v[u]) : 0u;
+                wb[u] = v[u] >= 0 ? bword(bmB, v[u]) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < SCAN_UNR; ++u)
+                if (v[u] >= 0) f(v[u], bbit(wa[u], v[u]), bbit(wb[u], v[u]), k + (long long)u * NT >= n1);
         }
     }
 
@@ -1180,15 +1210,13 @@ struct W {
         unsigned long long ob = P->oo[p], oe = P->oo[p + 1], ib = P->io[p], ie = P->io[p + 1];
         clip_live(P->on, ob, oe);
         clip_live(P->in, ib, ie);
-        wg_scan_bits(P->on, ob, oe, plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
-            if (live) {
+        wg_scan_bits2(P->on, ob, oe, P->in, ib, ie, plive, pmem, [&](int w, uint32_t live, uint32_t memb, bool in_list) {
+            if (!live) return;
+            if (!in_list) {
                 uint32_t ic = pic[w] - 1u;
                 pic[w] = ic;
                 if (pim[w] + ic < mi) { if (memb) fail = 1; else if (w == last) skip = 1; }
-            }
-        });
-        wg_scan_bits(P->in, ib, ie, plive, pmem, [&](int w, uint32_t live, uint32_t memb) {
-            if (live) {
+            } else {
                 uint32_t oc = poc[w] - 1u;
                 poc[w] = oc;
                 if (pom[w] + oc < mo) { if (memb) fail = 1; else if (w == last) skip = 1; }
@@ -1320,13 +1348,15 @@ struct W {
         dirty_l = nl;
         // reference src/device_kernels.cu:534-602 (d_add_one_vertex): v joins the members.
         // Members' degrees are updated; v's live candidate neighbours are marked.
-        wg_scan_bits(P->on, vob, voe, cmem, plive, [&](int w, uint32_t m, uint32_t l) {
-            if (m) { cim[w] += 1u; cic[w] -= 1u; }
-            else if (l) bs(ccrit, w);
-        });
-        wg_scan_bits(P->in, vib, vie, cmem, plive, [&](int w, uint32_t m, uint32_t l) {
-            if (m) { com[w] += 1u; coc[w] -= 1u; }
-            else if (l) bs(cL, w);
+        // (a vertex in both lists has its out- and in-fields updated by different threads)
+        wg_scan_bits2(P->on, vob, voe, P->in, vib, vie, cmem, plive, [&](int w, uint32_t m, uint32_t l, bool in_list) {
+            if (!in_list) {
+                if (m) { cim[w] += 1u; cic[w] -= 1u; }
+                else if (l) bs(ccrit, w);
+            } else {
+                if (m) { com[w] += 1u; coc[w] -= 1u; }
+                else if (l) bs(cL, w);
+            }
         });
         if (tid == 0) bytes += 4ull * ((P->oo[v + 1] - P->oo[v]) + (P->io[v + 1] - P->io[v])) + 32;
         __syncthreads();
@@ -1451,7 +1481,7 @@ struct W {
         tick(3);
         bool ok = degree_prune(cm, &R, &nr, tot - 1 - nm, ub, lbd, meo, mei, lb, lc);
         if (lazy_v >= 0) {  // (degree_prune materialized the degrees if it got past its first test)
-            clear_v_marks(lazy_v);
+            clear_v_marks();
             lazy_v = -1;
         }
         tick(4);
