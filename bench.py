@@ -44,9 +44,9 @@ WORKLOAD_NAME = ("R-MAT 2^20 directed, 2^24 arcs (.57/.19/.19/.05) + 500 planted
 # --pmc passes over this command (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
 # 256 MiB copy reads on the same counters); profiles/expand_kernel_rmat20_unit.md. It is a
 # measurement of this build, not of the run that prints it.
-PMC_TRAFFIC_PER_STEP = 1018.6e9
+PMC_TRAFFIC_PER_STEP = 425.0e9
 PMC_TRAFFIC_SOURCE = ("profiles/expand_kernel_rmat20_unit.md: rocprofv3 --pmc FETCH_SIZE (x2.0) and "
-                      "WRITE_SIZE (x1.0) passes over this unit; 731.3 GB fetched + 287.3 GB written per step")
+                      "WRITE_SIZE (x1.0) passes over this unit; 355.7 GB fetched + 69.3 GB written per step")
 
 
 def unit_config(dq, sched):
