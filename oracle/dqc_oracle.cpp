@@ -42,6 +42,9 @@ struct Graph {
     int64_t m = 0;
     std::vector<uint64_t> oo, io, to;  // offsets
     std::vector<int> on, in, tn;       // neighbours
+    // restricted two-hop build only: each source vertex's whole two-hop list length (the
+    // lists themselves keep only the flagged entries); empty after a full build
+    std::vector<uint32_t> l2full;
 };
 
 // ---------------------------------------------------------------------------
@@ -111,10 +114,16 @@ int load_text(const char* path, Graph& g) {
 //     some u1 in O*uB: u1->x, some u2 in O*uB: x->u2, some u3 in I*uB: u3->x,
 //     some u4 in I*uB: x->u4}; sorted ascending.
 // Otherwise L(v) is empty.
-void build_twohop(Graph& g, int threads) {
+// flag (restricted build): only flagged vertices get a list, it keeps only flagged
+// entries, and g.l2full[v] records the whole list's length. Level 0 reads no list and no
+// length of an unflagged vertex when the flags are its degree test (restricted_flags), so
+// it gives the full build's result from far fewer entries.
+void build_twohop(Graph& g, int threads, const std::vector<uint8_t>* flag = nullptr) {
     const int n = g.n;
     std::vector<std::vector<int>> lists(n);
     if (threads < 1) threads = 1;
+    if (flag) g.l2full.assign(n, 0);
+    else g.l2full.clear();
 #pragma omp parallel num_threads(threads)
     {
         std::vector<int> st_out(n, 0), st_in(n, 0);  // round stamps of O / I members
@@ -124,6 +133,7 @@ void build_twohop(Graph& g, int threads) {
         for (int v = 0; v < n; ++v) {
             uint64_t ob = g.oo[v], oe = g.oo[v + 1], ib = g.io[v], ie = g.io[v + 1];
             if (oe == ob || ie == ib) continue;
+            if (flag && !(*flag)[v]) continue;
             B.clear(); O.clear(); I.clear();
             for (uint64_t k = ob; k < oe; ++k) isout[g.on[k]] = 1;
             for (uint64_t k = ib; k < ie; ++k) isin[g.in[k]] = 1;
@@ -199,6 +209,10 @@ void build_twohop(Graph& g, int threads) {
             });
             for (int x : touched) mark[x] = 0;
             for (int x : L) inl[x] = 0;
+            if (flag) {
+                g.l2full[v] = (uint32_t)L.size();
+                L.erase(std::remove_if(L.begin(), L.end(), [&](int x) { return !(*flag)[x]; }), L.end());
+            }
             std::sort(L.begin(), L.end());
             lists[v] = L;
         }
@@ -207,6 +221,21 @@ void build_twohop(Graph& g, int threads) {
     for (int v = 0; v < n; ++v) g.to[v + 1] = g.to[v] + lists[v].size();
     g.tn.resize(g.to[n]);
     for (int v = 0; v < n; ++v) std::copy(lists[v].begin(), lists[v].end(), g.tn.begin() + g.to[v]);
+}
+
+// Two-hop lists for level 0 (mo, mi: the min-degree thresholds at min_size): all of them,
+// or with g_twohop_restricted only those of the vertices passing level 0's degree test,
+// holding only such vertices (the product's host build does the same on its search path).
+static bool g_twohop_restricted = false;
+void build_twohop_for_level0(Graph& g, int threads, int mo, int mi) {
+    if (!g_twohop_restricted) {
+        build_twohop(g, threads);
+        return;
+    }
+    std::vector<uint8_t> flag((size_t)g.n);
+    for (int v = 0; v < g.n; ++v)
+        flag[v] = (g.oo[v + 1] - g.oo[v] >= (uint64_t)mo) && (g.io[v + 1] - g.io[v] >= (uint64_t)mi);
+    build_twohop(g, threads, &flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,7 +699,7 @@ void level0(Graph& g, const Params& P, std::vector<Rec>& root, std::vector<int>&
         v[i].om = 0; v[i].im = 0;
         v[i].oc = (uint32_t)(g.oo[i + 1] - g.oo[i]);
         v[i].ic = (uint32_t)(g.io[i + 1] - g.io[i]);
-        v[i].l2 = (uint32_t)(g.to[i + 1] - g.to[i]);
+        v[i].l2 = g.l2full.empty() ? (uint32_t)(g.to[i + 1] - g.to[i]) : g.l2full[i];
         if (v[i].oc >= (unsigned)mo && v[i].ic >= (unsigned)mi && v[i].l2 >= (unsigned)(P.min - 1)) {
             v[i].label = 0; rem.push_back(i);
         } else {
@@ -820,11 +849,11 @@ int64_t orc_run(const char* graph_path, double gout, double gin, int min_size, i
     S.n_input = g.n;
     S.m_input = g.m;
     if (getenv("ORC_VERBOSE")) std::fprintf(stderr, "oracle: load %.2fs n=%d m=%lld\n", t1 - t0, g.n, (long long)g.m);
-    build_twohop(g, threads);
-    double t2 = now_s();
-    if (getenv("ORC_VERBOSE")) std::fprintf(stderr, "oracle: twohop %.2fs len=%zu\n", t2 - t1, g.tn.size());
     int len = 2 * (std::max(g.n, min_size) + 2) + 2;
     std::vector<int> mdo = mindeg_table(gout, len), mdi = mindeg_table(gin, len);
+    build_twohop_for_level0(g, threads, mdo[min_size], mdi[min_size]);
+    double t2 = now_s();
+    if (getenv("ORC_VERBOSE")) std::fprintf(stderr, "oracle: twohop %.2fs len=%zu\n", t2 - t1, g.tn.size());
     Params P{mdo.data(), mdi.data(), gout, gin, min_size, host_addfail_check != 0};
     std::vector<Rec> root;
     std::vector<int> orig_of;
@@ -966,6 +995,28 @@ int orc_graph(const char* path, int with_twohop, int threads, int* n_out, int64_
     return 0;
 }
 
+// Restricted two-hop build (build_twohop with the degree flags out >= mo, in >= mi) as
+// CSR plus each vertex's whole list length; arrays are malloc'd. Returns 0 or <0.
+int orc_twohop_restricted(const char* path, int mo, int mi, int threads, int* n_out, int64_t** to, int** tn,
+                          uint32_t** l2full) {
+    using namespace orc;
+    Graph g;
+    int rc = load_text(path, g);
+    if (rc) return rc;
+    std::vector<uint8_t> flag((size_t)g.n);
+    for (int v = 0; v < g.n; ++v)
+        flag[v] = (g.oo[v + 1] - g.oo[v] >= (uint64_t)mo) && (g.io[v + 1] - g.io[v] >= (uint64_t)mi);
+    build_twohop(g, threads < 1 ? 1 : threads, &flag);
+    *n_out = g.n;
+    *to = (int64_t*)std::malloc(sizeof(int64_t) * (g.to.size() + 1));
+    for (size_t i = 0; i < g.to.size(); ++i) (*to)[i] = (int64_t)g.to[i];
+    *tn = (int*)std::malloc(sizeof(int) * (g.tn.size() + 1));
+    if (!g.tn.empty()) std::memcpy(*tn, g.tn.data(), sizeof(int) * g.tn.size());
+    *l2full = (uint32_t*)std::malloc(sizeof(uint32_t) * (g.l2full.size() + 1));
+    if (!g.l2full.empty()) std::memcpy(*l2full, g.l2full.data(), sizeof(uint32_t) * g.l2full.size());
+    return 0;
+}
+
 // Level 0 + condense map: writes orig_of (input id of each condensed id) into a
 // malloc'd array; returns the condensed count or <0.
 int orc_condense_map(const char* path, double gout, double gin, int min_size, int threads, int** orig_of) {
@@ -973,9 +1024,9 @@ int orc_condense_map(const char* path, double gout, double gin, int min_size, in
     Graph g;
     int rc = load_text(path, g);
     if (rc) return rc;
-    build_twohop(g, threads < 1 ? 1 : threads);
     int len = 2 * (std::max(g.n, min_size) + 2) + 2;
     std::vector<int> mdo = mindeg_table(gout, len), mdi = mindeg_table(gin, len);
+    build_twohop_for_level0(g, threads < 1 ? 1 : threads, mdo[min_size], mdi[min_size]);
     Params P{mdo.data(), mdi.data(), gout, gin, min_size, true};
     std::vector<Rec> root;
     std::vector<int> oo;
@@ -986,6 +1037,10 @@ int orc_condense_map(const char* path, double gout, double gin, int min_size, in
 }
 
 void orc_free(void* p) { std::free(p); }
+
+// on != 0: later orc_run / orc_condense_map / orc_open build only the two-hop lists level 0
+// can read (build_twohop_for_level0); the results are those of the full build.
+void orc_set_twohop_restricted(int on) { orc::g_twohop_restricted = on != 0; }
 
 int orc_mindeg(double gamma, int len, int* out) {
     auto md = orc::mindeg_table(gamma, len);
@@ -1041,10 +1096,10 @@ void* orc_open(const char* path, double gout, double gin, int min_size, int thre
     using namespace orc;
     auto* h = new orc_handle();
     if (load_text(path, h->g) != 0) { delete h; return nullptr; }
-    build_twohop(h->g, threads < 1 ? 1 : threads);
     int len = 2 * (std::max(h->g.n, min_size) + 2) + 2;
     h->mdo = mindeg_table(gout, len);
     h->mdi = mindeg_table(gin, len);
+    build_twohop_for_level0(h->g, threads < 1 ? 1 : threads, h->mdo[min_size], h->mdi[min_size]);
     h->P = Params{h->mdo.data(), h->mdi.data(), gout, gin, min_size, host_addfail_check != 0};
     std::vector<int> orig;
     level0(h->g, h->P, h->root, orig);

@@ -48,6 +48,11 @@ def load():
     lib.orc_condense_map.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(P32)]
     lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_set_twohop_restricted.argtypes = [ctypes.c_int]
+    lib.orc_twohop_restricted.restype = ctypes.c_int
+    lib.orc_twohop_restricted.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P32,
+                                          ctypes.POINTER(P64), ctypes.POINTER(P32),
+                                          ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))]
     lib.orc_mindeg.argtypes = [ctypes.c_double, ctypes.c_int, P32]
     lib.orc_rmnonmax.restype = ctypes.c_int
     lib.orc_rmnonmax.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
@@ -68,6 +73,33 @@ def load():
 
 def _b(p):
     return None if p is None else os.fsencode(p)
+
+
+def set_twohop_restricted(on):
+    """Later run / condense_map / Tasks build only the two-hop lists level 0 can read (those
+    of the vertices passing its degree test, holding only such vertices); same results."""
+    load().orc_set_twohop_restricted(int(bool(on)))
+
+
+def twohop_restricted(path, mo, mi, threads=1):
+    """Restricted two-hop build with degree flags out >= mo, in >= mi:
+    (n, offsets, entries, whole list length per vertex)."""
+    lib = load()
+    n = ctypes.c_int()
+    to = ctypes.POINTER(ctypes.c_int64)()
+    tn = ctypes.POINTER(ctypes.c_int)()
+    l2 = ctypes.POINTER(ctypes.c_uint32)()
+    rc = lib.orc_twohop_restricted(_b(path), int(mo), int(mi), threads, ctypes.byref(n), ctypes.byref(to),
+                                   ctypes.byref(tn), ctypes.byref(l2))
+    if rc != 0:
+        raise RuntimeError(f"orc_twohop_restricted failed: {rc}")
+    N = n.value
+    offs = [to[i] for i in range(N + 1)]
+    ent = [tn[i] for i in range(offs[N])]
+    full = [l2[i] for i in range(N)]
+    for p in (to, tn, l2):
+        lib.orc_free(p)
+    return N, offs, ent, full
 
 
 def run(graph_path, out_gamma, in_gamma, min_size, raw_path=None, max_path=None, threads=0,

@@ -95,6 +95,54 @@ def test_twohop_vs_restatement(oracle_lib, tmp_path):
                 assert set(arcs[v]) & set(inn[v]) <= set(L)
 
 
+def test_restricted_twohop_vs_full(oracle_lib, tmp_path):
+    """The restricted two-hop build (the lists level 0 can read: those of the vertices that
+    pass its degree test, holding only such vertices) against the full build, itself pinned
+    above: each flagged vertex's list is its full list cut to flagged vertices and its
+    recorded length is the full list's; unflagged vertices get nothing."""
+    rng = random.Random(11)
+    for t in range(40):
+        n = rng.randint(2, 40)
+        p = rng.choice([0.1, 0.2, 0.35, 0.5])
+        arcs = [sorted(v for v in range(n) if v != u and rng.random() < p) for u in range(n)]
+        path = write(tmp_path, f"r{t}.txt", "".join(" ".join(map(str, a)) + "\n" for a in arcs))
+        full = oracle_lib.graph(path, with_twohop=True)
+        mo, mi = rng.randint(0, 4), rng.randint(0, 4)
+        N, offs, ent, l2 = oracle_lib.twohop_restricted(path, mo, mi)
+        assert N == n
+        flag = [len(full["out"][v]) >= mo and len(full["in"][v]) >= mi for v in range(n)]
+        for v in range(n):
+            got = ent[offs[v]:offs[v + 1]]
+            if flag[v]:
+                assert got == [x for x in full["two"][v] if flag[x]], (t, v)
+                assert l2[v] == len(full["two"][v]), (t, v)
+            else:
+                assert got == [] and l2[v] == 0, (t, v)
+
+
+def test_restricted_twohop_same_results(oracle_lib, manifest, tmp_path):
+    """The whole method on the golden cases with the restricted two-hop build: the same
+    maximal files and tree counts as the full build's committed expectations, and the same
+    level-0 condensation."""
+    from tests.conftest import GOLDEN
+    oracle_lib.set_twohop_restricted(True)
+    try:
+        for c in manifest["cases"][:20]:
+            g = os.path.join(GOLDEN, c["graph"])
+            mx = str(tmp_path / (c["name"] + ".r.max"))
+            r, st = oracle_lib.run(g, c["out_gamma"], c["in_gamma"], c["min_size"], None, mx, threads=2)
+            assert r == c["maximal"] and st.tasks == c["tasks"] and st.branches == c["branches"], c["name"]
+            if c["expected"]:
+                assert open(mx, "rb").read() == open(os.path.join(GOLDEN, c["expected"]), "rb").read()
+            restricted_map = oracle_lib.condense_map(g, c["out_gamma"], c["in_gamma"], c["min_size"])
+            oracle_lib.set_twohop_restricted(False)
+            full_map = oracle_lib.condense_map(g, c["out_gamma"], c["in_gamma"], c["min_size"])
+            oracle_lib.set_twohop_restricted(True)
+            assert restricted_map == full_map, c["name"]
+    finally:
+        oracle_lib.set_twohop_restricted(False)
+
+
 def test_rmnonmax_vs_subset_filter(oracle_lib, tmp_path):
     rng = random.Random(11)
     for t in range(40):
