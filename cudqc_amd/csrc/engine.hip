@@ -101,22 +101,14 @@ __host__ __device__ inline bool split_eligible(const TaskDesc& d, int split_b) {
     return split_b > 0 && !(d.flags & TF_PIECE) && d.mode != MODE_HOST_L0 && steps >= split_b;
 }
 // Pieces for a split task of estimated cost c in a batch of total estimated cost C: the
-// batch's work cut into about `fill` pieces per workgroup of similar cost
-// (C / (fill x workgroups) each), so a task gets pieces in proportion to its cost; at least
-// 2, at most max_pieces.
-__host__ __device__ inline int pieces_for(double c, double total, int workgroups, int max_pieces, int fill) {
-    const double target = total / ((double)fill * (double)(workgroups > 0 ? workgroups : 1));
+// batch's work cut into about four pieces per workgroup of similar cost (C / (4 x
+// workgroups) each), so a task gets pieces in proportion to its cost; at least 2, at most
+// max_pieces.
+__host__ __device__ inline int pieces_for(double c, double total, int workgroups, int max_pieces) {
+    const double target = total / (4.0 * (double)(workgroups > 0 ? workgroups : 1));
     double p = target > 0.0 ? ceil(c / target) : (double)max_pieces;
     p = fmin(fmax(p, 2.0), (double)(max_pieces > 2 ? max_pieces : 2));
     return (int)p;
-}
-// Pieces per workgroup for a batch with n_elig tasks that may be split. Pieces' real costs
-// differ more than the estimate says (a piece whose branches pass the bound tests runs
-// degree recounts), so with few such tasks many short pieces (16 per workgroup) keep the
-// launch's tail short. With many, 4: their pieces' records would exhaust the split budget
-// and leave later tasks unsplit.
-__host__ __device__ inline int piece_fill(unsigned long long n_elig, int workgroups) {
-    return n_elig * 16ull <= (unsigned long long)(workgroups > 0 ? workgroups : 1) ? 16 : 4;
 }
 // Steps per piece when a task of `steps` loop steps is split: as many pieces as
 // max_pieces allows, each of at least split_c steps (host and device schedulers both
@@ -1959,12 +1951,11 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
     __shared__ unsigned long long scan[POP_T];
     __shared__ unsigned long long carry;
     __shared__ int any_long;
-    __shared__ unsigned n_elig;
     __shared__ double csum[POP_T];
     const unsigned long long n = q->ntask, first = q->first, budget = q->split_budget;
     const int tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
-    if (tid == 0) { carry = 0; any_long = 0; n_elig = 0; }
+    if (tid == 0) { carry = 0; any_long = 0; }
     __syncthreads();
     auto cost = [&](unsigned long long i) {
         const TaskDesc d = sdesc[first + i];
@@ -1985,7 +1976,6 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
         part += cost(i);
         const TaskDesc d = sdesc[first + i];
         if (task_steps(d) >= long_task_steps(split_b)) any_long = 1;
-        if (split_eligible(d, split_b)) atomicAdd(&n_elig, 1u);
     }
     csum[tid] = part;
     __syncthreads();
@@ -1994,7 +1984,6 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
         __syncthreads();
     }
     const double total = csum[0];
-    const int fill = piece_fill(n_elig, workgroups);
     if (!any_long) {  // no task long enough to split or to unbalance the launch: stack order
         for (unsigned long long i = tid; i < n; i += POP_T) { perm[i] = (uint32_t)i; split[i] = 0; }
         return;
@@ -2045,7 +2034,7 @@ __global__ __launch_bounds__(POP_T) void order_kernel(const TaskDesc* __restrict
             const TaskDesc d = sdesc[first + perm[k]];
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (split_eligible(d, split_b)) {
-                chunk = split_chunk_for(steps, split_c, pieces_for(cost(perm[k]), total, workgroups, max_pieces, fill));
+                chunk = split_chunk_for(steps, split_c, pieces_for(cost(perm[k]), total, workgroups, max_pieces));
                 need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
             }
         }
@@ -2795,18 +2784,13 @@ static dqc_status pop_batch(dqc_handle* h, int32_t sched, unsigned long long* po
         const unsigned long long used = h->st_v + h->def_v;
         unsigned long long budget = h->cfg.spill_vertices > used ? (h->cfg.spill_vertices - used) / 4 : 0;
         double total = 0.0;
-        unsigned long long n_elig = 0;
-        for (unsigned long long i = 0; i < cnt; ++i) {
-            total += h->hcost[i];
-            if (split_eligible(b[i], h->cfg.split_branches)) ++n_elig;
-        }
-        const int fill = piece_fill(n_elig, h->launch_wg);
+        for (unsigned long long i = 0; i < cnt; ++i) total += h->hcost[i];
         for (unsigned long long i = 0; i < cnt; ++i) {
             TaskDesc d = b[order[i]];
             d.off -= v0;
             const int steps = d.tot - d.nm - (d.mode == MODE_DEVICE ? 0 : d.ncov);
             if (split_eligible(d, h->cfg.split_branches)) {
-                const int pieces = pieces_for(h->hcost[order[i]], total, h->launch_wg, max_pieces(h), fill);
+                const int pieces = pieces_for(h->hcost[order[i]], total, h->launch_wg, max_pieces(h));
                 const int chunk = split_chunk_for(steps, h->cfg.split_chunk, pieces);
                 const unsigned long long need = (unsigned long long)((steps + chunk - 1) / chunk) * (unsigned long long)d.tot;
                 if (need <= budget) {

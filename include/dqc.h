@@ -63,10 +63,8 @@ typedef struct {
                                   0 = all                                             */
     int32_t split_pieces;      /* most pieces one split task is cut into; 0 = as many
                                   as a launch has workgroups. A batch with several
-                                  tasks to split cuts their work into about F x
-                                  workgroups pieces in proportion to their estimated
-                                  costs (F = 16 with at most workgroups / 16 tasks to
-                                  split, else 4), so the batch fills a launch         */
+                                  tasks to split gives each about 4 x workgroups /
+                                  (their number) pieces, so the batch fills a launch  */
     int32_t root_first;        /* > 0: the unit starts at level-0 step root_first. The
                                   steps before it run only until the first one that
                                   writes a child (that child keeps the host level-1
