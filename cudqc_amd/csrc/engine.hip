@@ -1372,13 +1372,13 @@ struct W {
         clear_hist(cm);
         __syncthreads();
     }
-    // Clears the child's bitmap words (every bit set in a branch belongs to one of its
-    // members or L(v) candidates), so the bitmaps are all-zero between branches, also
-    // across launches when they live in HBM.
+    // Clears the child's bitmap words, so the bitmaps are all-zero between branches, also
+    // across launches when they live in HBM. The marks of v's neighbours (cL, ccrit) are
+    // already cleared when the branch's degree test ends; every other bit set in a branch
+    // belongs to one of its members or L(v) candidates: member bits of mem[0..dirty_m),
+    // candidate bits among lst, and the members and ccrit bits critical() sets among lst.
     __device__ __forceinline__ void end_child() {
         __syncthreads();
-        // (cL and v's ccrit marks are cleared when the branch's degree test ends; members
-        // other than those in mem[0..dirty_m) and ccrit bits come from critical())
         const int nclear = dirty_m + (cand_marks || crit_marks ? dirty_l : 0);
         for (int i = tid; i < nclear; i += NT) {
             if (i < dirty_m) {
